@@ -1,0 +1,137 @@
+/*
+ * zk_gpu_amd — BN256 Fr number-theoretic transform on AMD MI355X (gfx950).
+ *
+ * C-ABI of libzkgpu_ntt.so. Plain pointers and sizes only.
+ *
+ * Field elements: BN256 scalar field Fr, r = 0x30644e72e131a029b85045b68181585d
+ * 2833e84879b9709143e1f593f0000001, in Montgomery form x * 2^256 mod r, as four
+ * little-endian uint64 words per element (halo2's Fr layout; the reference's
+ * BigNum, cuda/bn256.h:10-13). Inputs must be canonical (< r); outputs are
+ * canonical and bit-identical to halo2's best_fft on the same inputs.
+ *
+ * Return codes: 0 on success, otherwise one of ZKGPU_ERR_*. No entry point
+ * prints or exits (the reference prints and calls exit() on any CUDA error,
+ * cuda/fft.cu:379-395).
+ */
+#ifndef ZKGPU_NTT_H
+#define ZKGPU_NTT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ZKGPU_OK 0
+#define ZKGPU_ERR_INVALID 1      /* bad argument (size, pointer, op, device) */
+#define ZKGPU_ERR_UNSUPPORTED 2  /* entry point not provided (MSM) */
+#define ZKGPU_ERR_HIP 3          /* HIP runtime error */
+#define ZKGPU_ERR_NOMEM 4        /* device allocation failed */
+#define ZKGPU_ERR_NONCANONICAL 5 /* an input element is >= r (validation enabled) */
+#define ZKGPU_ERR_BUDGET 6       /* a job exceeds the serving HBM budget */
+#define ZKGPU_ERR_SHUTDOWN 7     /* the server is stopping */
+
+#define ZKGPU_OP_FWD 0       /* X_k = sum_i a_i w^(ik) */
+#define ZKGPU_OP_INV 1       /* X_k = divisor * sum_i a_i w^(ik)  (w = omega^-1 for halo2's ifft) */
+#define ZKGPU_OP_COSET_FWD 2 /* X_k = sum_i a_i c^i w^(ik) */
+#define ZKGPU_OP_COSET_INV 3 /* X_k = divisor * c^k * sum_i a_i w^(ik)  (the inverse of COSET_FWD with c -> c^-1) */
+
+/* ---------------------------------------------------------------------
+ * Drop-in entry points of the reference FFI (src/api.rs:3-13,
+ * implemented there by cuda/fft.cu:553-584). Same names and argument
+ * meaning; widths follow the Rust declarations (usize / u64), which is what
+ * the Rust caller passes. Host buffers, synchronous, data transformed in
+ * place. `twiddle`/`twiddle_size` (the caller-built lookup table,
+ * src/main.rs:128-138) are accepted and ignored: the library keeps its own
+ * tables resident on the device, keyed by omega. n must equal 2^log_n,
+ * 1 <= log_n <= 28 (the reference requires log_n >= 10).
+ * --------------------------------------------------------------------- */
+
+/* replaces do_fft_ffi (src/api.rs:4, cuda/fft.cu:554-558): device smoke
+ * check; runs a 2^10 transform round trip on the device, 0 when it holds. */
+uint64_t do_fft_ffi(size_t device_id);
+
+/* replaces parallel_fft_api (src/api.rs:6, cuda/fft.cu:560-564):
+ * req <- FFT_omega(req) (halo2 best_fft). */
+uint64_t parallel_fft_api(size_t device_id, uint64_t* req, const uint64_t* omega, const uint64_t* twiddle,
+                          uint64_t n, uint64_t log_n, uint64_t twiddle_size);
+
+/* replaces parallel_ifft_api (src/api.rs:8, cuda/fft.cu:566-570):
+ * req <- divisor * FFT_omega(req); the caller passes omega^-1 and n^-1. */
+uint64_t parallel_ifft_api(size_t device_id, uint64_t* req, const uint64_t* omega, const uint64_t* twiddle,
+                           uint64_t n, uint64_t log_n, uint64_t twiddle_size, const uint64_t* divisor);
+
+/* replaces parallel_coeff_to_extended_part_api (src/api.rs:10,
+ * cuda/fft.cu:572-576): req[i] *= c^i, then req <- FFT_omega(req). */
+uint64_t parallel_coeff_to_extended_part_api(size_t device_id, uint64_t* req, const uint64_t* omega,
+                                             const uint64_t* twiddle, uint64_t n, uint64_t log_n,
+                                             uint64_t twiddle_size, const uint64_t* c);
+
+/* replaces parallel_msm_api (src/api.rs:12, cuda/fft.cu:578-582): the G1 MSM
+ * is outside this library; always returns ZKGPU_ERR_UNSUPPORTED. */
+uint64_t parallel_msm_api(size_t device_id, const uint64_t* data, const void* bases, uint64_t length, void* acc);
+
+/* ---------------------------------------------------------------------
+ * Extensions (no counterpart in the reference FFI).
+ * --------------------------------------------------------------------- */
+
+/* Coset inverse on host buffers: req <- divisor * c^k * FFT_omega(req)[k];
+ * with omega^-1, n^-1, g^-1 it inverts parallel_coeff_to_extended_part_api
+ * (coset g). Same size rules as the drop-in entry points. */
+uint64_t zkgpu_coset_ifft_api(size_t device_id, uint64_t* req, const uint64_t* omega, uint64_t n,
+                              uint64_t log_n, const uint64_t* divisor, const uint64_t* c);
+
+/* Device-resident transform, asynchronous on `stream` (a hipStream_t, NULL
+ * for the null stream). d_in and d_out are device pointers to 2^log_n
+ * elements; they may be equal (in place). omega is required; divisor is
+ * required for OP_INV and OP_COSET_INV; c for the coset ops (host pointers to
+ * one element each, read before the call returns). */
+int zkgpu_ntt_device(int device_id, const void* d_in, void* d_out, uint32_t log_n, int op,
+                     const uint64_t* omega, const uint64_t* divisor, const uint64_t* c, void* stream);
+
+/* Device-resident transform with halo2's standard domain of size 2^log_n:
+ * omega = 7^((r-1)/2^28) squared down to order 2^log_n, divisor = n^-1 and
+ * coset generator g = 7 (OP_COSET_INV uses g^-1, so it inverts OP_COSET_FWD). */
+int zkgpu_ntt_std_device(int device_id, const void* d_in, void* d_out, uint32_t log_n, int op, void* stream);
+
+/* Domain constants of the standard domain (Montgomery words): which = 0 omega,
+ * 1 omega^-1, 2 n^-1, 3 g, 4 g^-1. */
+int zkgpu_domain_constant(uint32_t log_n, int which, uint64_t* out4);
+
+/* Build (or fetch) every resident table a transform needs, so the first
+ * timed call does not pay for it. Arguments as zkgpu_ntt_device. */
+int zkgpu_prepare(int device_id, uint32_t log_n, int op, const uint64_t* omega, const uint64_t* divisor,
+                  const uint64_t* c);
+
+/* Seeded synthetic canonical elements: element i of d_out gets synthetic
+ * element (start + i) of `seed` (SplitMix64, rejection below r, Montgomery
+ * encoded). Asynchronous on `stream`. */
+int zkgpu_fill_random(int device_id, void* d_out, uint64_t count, uint64_t seed, uint64_t start, void* stream);
+
+/* Set *d_flag (device uint32) to nonzero when some element is >= r. */
+int zkgpu_check_canonical(int device_id, const void* d_in, uint64_t count, uint32_t* d_flag, void* stream);
+
+/* Pass split used for 2^log_n: fills digits[3] (log2 DFT length per pass)
+ * and log_cols[3]; returns the number of passes, or 0 when log_n is out of
+ * range. Host only. */
+int zkgpu_plan_get(uint32_t log_n, uint32_t* digits, uint32_t* log_cols);
+/* Install a pass split for 2^log_n (the autotuner's choice); returns
+ * ZKGPU_ERR_INVALID for an inconsistent split. npass = 0 restores the default. */
+int zkgpu_plan_set(uint32_t log_n, uint32_t npass, const uint32_t* digits, const uint32_t* log_cols);
+
+/* Resident table bytes on a device; drop them with zkgpu_drop_tables. */
+uint64_t zkgpu_table_bytes(int device_id);
+int zkgpu_drop_tables(int device_id);
+
+/* Text of a return code. */
+const char* zkgpu_error_string(int code);
+
+/* Library version, major * 10000 + minor * 100 + patch. */
+int zkgpu_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ZKGPU_NTT_H */

@@ -1,0 +1,200 @@
+"""zk_gpu_amd: BN256 Fr NTT on AMD MI355X (gfx950).
+
+Python view of libzkgpu_ntt.so (C-ABI declared in include/zkgpu_ntt.h). The
+functions below mirror the reference's FFI (src/api.rs:3-13): same names,
+same argument meaning, 0 on success and a nonzero code otherwise. Host
+buffers are numpy uint64 arrays of shape (n, 4) (four little-endian words
+per element, Montgomery form x * 2^256 mod r). Device buffers are raw
+device pointers (ints), e.g. torch tensors' data_ptr().
+
+The library must be built (python -m zk_gpu_amd.build); importing this
+module without it raises: there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libzkgpu_ntt.so")
+
+OK = 0
+ERR_INVALID = 1
+ERR_UNSUPPORTED = 2
+ERR_HIP = 3
+ERR_NOMEM = 4
+ERR_NONCANONICAL = 5
+ERR_BUDGET = 6
+ERR_SHUTDOWN = 7
+
+OP_FWD = 0
+OP_INV = 1
+OP_COSET_FWD = 2
+OP_COSET_INV = 3
+OPS = {"fwd": OP_FWD, "inv": OP_INV, "coset_fwd": OP_COSET_FWD, "coset_inv": OP_COSET_INV}
+
+_P = ctypes.c_void_p
+_U64 = ctypes.c_uint64
+_U32 = ctypes.c_uint32
+_I = ctypes.c_int
+_SZ = ctypes.c_size_t
+
+# name: (restype, argtypes)
+SIGNATURES = {
+    "do_fft_ffi": (_U64, [_SZ]),
+    "parallel_fft_api": (_U64, [_SZ, _P, _P, _P, _U64, _U64, _U64]),
+    "parallel_ifft_api": (_U64, [_SZ, _P, _P, _P, _U64, _U64, _U64, _P]),
+    "parallel_coeff_to_extended_part_api": (_U64, [_SZ, _P, _P, _P, _U64, _U64, _U64, _P]),
+    "parallel_msm_api": (_U64, [_SZ, _P, _P, _U64, _P]),
+    "zkgpu_coset_ifft_api": (_U64, [_SZ, _P, _P, _U64, _U64, _P, _P]),
+    "zkgpu_ntt_device": (_I, [_I, _P, _P, _U32, _I, _P, _P, _P, _P]),
+    "zkgpu_ntt_std_device": (_I, [_I, _P, _P, _U32, _I, _P]),
+    "zkgpu_domain_constant": (_I, [_U32, _I, _P]),
+    "zkgpu_prepare": (_I, [_I, _U32, _I, _P, _P, _P]),
+    "zkgpu_fill_random": (_I, [_I, _P, _U64, _U64, _U64, _P]),
+    "zkgpu_check_canonical": (_I, [_I, _P, _U64, _P, _P]),
+    "zkgpu_plan_get": (_I, [_U32, _P, _P]),
+    "zkgpu_plan_set": (_I, [_U32, _U32, _P, _P]),
+    "zkgpu_table_bytes": (_U64, [_I]),
+    "zkgpu_drop_tables": (_I, [_I]),
+    "zkgpu_error_string": (ctypes.c_char_p, [_I]),
+    "zkgpu_version": (_I, []),
+}
+OPTIONAL = {}
+
+_lib = None
+
+
+def lib():
+    """The loaded library (raises when it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m zk_gpu_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in {**SIGNATURES, **OPTIONAL}.items():
+            if not hasattr(L, name):
+                if name in OPTIONAL:
+                    continue
+                raise RuntimeError(f"{LIB_PATH} does not export {name}")
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+class ZkGpuError(RuntimeError):
+    def __init__(self, code, what=""):
+        self.code = code
+        msg = lib().zkgpu_error_string(int(code)).decode()
+        super().__init__(f"{what}: {msg} (code {code})" if what else f"{msg} (code {code})")
+
+
+def check(code, what=""):
+    if int(code) != OK:
+        raise ZkGpuError(int(code), what)
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if isinstance(a, int):
+        return a
+    assert a.dtype == np.uint64 and a.flags["C_CONTIGUOUS"], "expected a C-contiguous uint64 array"
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _elem(a):
+    if a is None:
+        return None
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.uint64).reshape(4))
+    return a
+
+
+# ---------------------------------------------------------------- reference FFI mirror
+
+def do_fft_ffi(device_id=0):
+    return int(lib().do_fft_ffi(device_id))
+
+
+def parallel_fft_api(device_id, req, omega, twiddle, n, log_n, twiddle_size):
+    """req (numpy (n,4) uint64) <- FFT_omega(req), in place (reference cuda/fft.cu:560-564)."""
+    om = _elem(omega)
+    return int(lib().parallel_fft_api(device_id, _ptr(req), _ptr(om), _ptr(twiddle), n, log_n, twiddle_size))
+
+
+def parallel_ifft_api(device_id, req, omega, twiddle, n, log_n, twiddle_size, divisor):
+    """req <- divisor * FFT_omega(req) (reference cuda/fft.cu:566-570)."""
+    om, dv = _elem(omega), _elem(divisor)
+    return int(lib().parallel_ifft_api(device_id, _ptr(req), _ptr(om), _ptr(twiddle), n, log_n, twiddle_size,
+                                       _ptr(dv)))
+
+
+def parallel_coeff_to_extended_part_api(device_id, req, omega, twiddle, n, log_n, twiddle_size, c):
+    """req[i] *= c^i, then req <- FFT_omega(req) (reference cuda/fft.cu:572-576)."""
+    om, cc = _elem(omega), _elem(c)
+    return int(lib().parallel_coeff_to_extended_part_api(device_id, _ptr(req), _ptr(om), _ptr(twiddle), n, log_n,
+                                                         twiddle_size, _ptr(cc)))
+
+
+def parallel_msm_api(device_id, data, bases, length, acc):
+    return int(lib().parallel_msm_api(device_id, _ptr(data), bases, length, acc))
+
+
+def zkgpu_coset_ifft_api(device_id, req, omega, n, log_n, divisor, c):
+    om, dv, cc = _elem(omega), _elem(divisor), _elem(c)
+    return int(lib().zkgpu_coset_ifft_api(device_id, _ptr(req), _ptr(om), n, log_n, _ptr(dv), _ptr(cc)))
+
+
+# ---------------------------------------------------------------- device-resident API
+
+def domain_constant(log_n, which):
+    """which: 0 omega, 1 omega^-1, 2 n^-1, 3 g=7, 4 g^-1 (Montgomery words)."""
+    out = np.zeros(4, dtype=np.uint64)
+    check(lib().zkgpu_domain_constant(log_n, which, _ptr(out)), "zkgpu_domain_constant")
+    return out
+
+
+def ntt_device(d_in, d_out, log_n, op, omega, divisor=None, c=None, stream=None, device_id=0):
+    om, dv, cc = _elem(omega), _elem(divisor), _elem(c)
+    check(lib().zkgpu_ntt_device(device_id, d_in, d_out, log_n, op, _ptr(om), _ptr(dv), _ptr(cc), stream),
+          "zkgpu_ntt_device")
+
+
+def ntt_std_device(d_in, d_out, log_n, op, stream=None, device_id=0):
+    check(lib().zkgpu_ntt_std_device(device_id, d_in, d_out, log_n, op, stream), "zkgpu_ntt_std_device")
+
+
+def prepare_std(log_n, op, device_id=0):
+    inverse = op in (OP_INV, OP_COSET_INV)
+    om = domain_constant(log_n, 1 if inverse else 0)
+    dv = domain_constant(log_n, 2)
+    cc = domain_constant(log_n, 4 if op == OP_COSET_INV else 3)
+    check(lib().zkgpu_prepare(device_id, log_n, op, _ptr(om), _ptr(dv), _ptr(cc)), "zkgpu_prepare")
+
+
+def fill_random(d_out, count, seed, start=0, stream=None, device_id=0):
+    check(lib().zkgpu_fill_random(device_id, d_out, count, seed, start, stream), "zkgpu_fill_random")
+
+
+def plan_get(log_n):
+    d = np.zeros(3, dtype=np.uint32)
+    c = np.zeros(3, dtype=np.uint32)
+    npass = lib().zkgpu_plan_get(log_n, d.ctypes.data_as(ctypes.c_void_p), c.ctypes.data_as(ctypes.c_void_p))
+    return int(npass), [int(x) for x in d[:npass]], [int(x) for x in c[:npass]]
+
+
+def plan_set(log_n, digits, log_cols):
+    d = np.array(list(digits) + [0] * (3 - len(digits)), dtype=np.uint32)
+    c = np.array(list(log_cols) + [0] * (3 - len(log_cols)), dtype=np.uint32)
+    check(lib().zkgpu_plan_set(log_n, len(digits), d.ctypes.data_as(ctypes.c_void_p),
+                               c.ctypes.data_as(ctypes.c_void_p)), "zkgpu_plan_set")
+
+
+def plan_reset(log_n):
+    check(lib().zkgpu_plan_set(log_n, 0, None, None), "zkgpu_plan_set")
+
+
+def version():
+    return int(lib().zkgpu_version())

@@ -1,0 +1,260 @@
+// extern "C" entry points of libzkgpu_ntt.so (declared in include/zkgpu_ntt.h).
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include "../../include/zkgpu_ntt.h"
+#include "engine.hpp"
+
+using namespace zkgpu;
+
+namespace {
+
+bool size_ok(uint64_t n, uint64_t log_n) {
+  return log_n >= 1 && log_n <= MAX_LOG_N && n == (1ull << log_n);
+}
+
+// One synchronous host-buffer transform: upload, transform in place on the
+// calling thread's stream, download. The reference does the same per call
+// with fresh allocations and device-wide syncs (cuda/fft.cu:275-405).
+int host_call(size_t device_id, uint64_t* req, uint64_t log_n, int op, const uint64_t* omega,
+              const uint64_t* divisor, const uint64_t* c) {
+  if (!req || !omega) return ZK_ERR_INVALID;
+  if ((op == OP_INV || op == OP_COSET_INV) && !divisor) return ZK_ERR_INVALID;
+  if ((op == OP_COSET_FWD || op == OP_COSET_INV) && !c) return ZK_ERR_INVALID;
+  int st = ZK_OK;
+  Engine* e = Engine::get((int)device_id, &st);
+  if (!e) return st;
+  hipStream_t s = e->thread_stream();
+  if (!s) return ZK_ERR_HIP;
+  const size_t bytes = (size_t)32 << log_n;
+  void* d = e->workspace(s, bytes, 1);
+  if (!d) return ZK_ERR_NOMEM;
+  JobSpec job;
+  job.in = d;
+  job.out = d;
+  job.log_n = (uint32_t)log_n;
+  job.op = op;
+  job.omega = host::from_ptr(omega);
+  if (divisor) job.divisor = host::from_ptr(divisor);
+  if (c) job.coset = host::from_ptr(c);
+  if (hipMemcpyAsync(d, req, bytes, hipMemcpyHostToDevice, s) != hipSuccess) return ZK_ERR_HIP;
+  st = e->run(job, s);
+  if (st) {
+    (void)hipStreamSynchronize(s);
+    return st;
+  }
+  if (hipMemcpyAsync(req, d, bytes, hipMemcpyDeviceToHost, s) != hipSuccess) return ZK_ERR_HIP;
+  if (hipStreamSynchronize(s) != hipSuccess) return ZK_ERR_HIP;
+  return ZK_OK;
+}
+
+host::Fr std_constant(uint32_t log_n, int which) {
+  switch (which) {
+    case 0: return host::root_of_unity(log_n);
+    case 1: return host::inv(host::root_of_unity(log_n));
+    case 2: return host::inv(host::from_u64(1ull << log_n));
+    case 3: return host::from_u64(7);
+    default: return host::inv(host::from_u64(7));
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+uint64_t do_fft_ffi(size_t device_id) {
+  // round trip of a 2^10 synthetic vector through forward and inverse
+  int st = ZK_OK;
+  Engine* e = Engine::get((int)device_id, &st);
+  if (!e) return (uint64_t)st;
+  hipStream_t s = e->thread_stream();
+  const uint32_t k = 10;
+  const size_t bytes = (size_t)32 << k;
+  void* a = e->workspace(s, 2 * bytes, 2);
+  if (!a) return ZK_ERR_NOMEM;
+  void* b = (char*)a + bytes;
+  if (launch_fill_random(a, 1, 0, 1ull << k, s) != hipSuccess) return ZK_ERR_HIP;
+  if (zkgpu_ntt_std_device((int)device_id, a, b, k, ZKGPU_OP_FWD, s)) return ZK_ERR_HIP;
+  if (zkgpu_ntt_std_device((int)device_id, b, b, k, ZKGPU_OP_INV, s)) return ZK_ERR_HIP;
+  std::vector<uint64_t> ha(4u << k), hb(4u << k);
+  if (hipMemcpyAsync(ha.data(), a, bytes, hipMemcpyDeviceToHost, s) != hipSuccess) return ZK_ERR_HIP;
+  if (hipMemcpyAsync(hb.data(), b, bytes, hipMemcpyDeviceToHost, s) != hipSuccess) return ZK_ERR_HIP;
+  if (hipStreamSynchronize(s) != hipSuccess) return ZK_ERR_HIP;
+  return ha == hb ? ZK_OK : ZK_ERR_HIP;
+}
+
+uint64_t parallel_fft_api(size_t device_id, uint64_t* req, const uint64_t* omega, const uint64_t* twiddle,
+                          uint64_t n, uint64_t log_n, uint64_t twiddle_size) {
+  (void)twiddle;
+  (void)twiddle_size;
+  if (!size_ok(n, log_n)) return ZK_ERR_INVALID;
+  return (uint64_t)host_call(device_id, req, log_n, OP_FWD, omega, nullptr, nullptr);
+}
+
+uint64_t parallel_ifft_api(size_t device_id, uint64_t* req, const uint64_t* omega, const uint64_t* twiddle,
+                           uint64_t n, uint64_t log_n, uint64_t twiddle_size, const uint64_t* divisor) {
+  (void)twiddle;
+  (void)twiddle_size;
+  if (!size_ok(n, log_n)) return ZK_ERR_INVALID;
+  return (uint64_t)host_call(device_id, req, log_n, OP_INV, omega, divisor, nullptr);
+}
+
+uint64_t parallel_coeff_to_extended_part_api(size_t device_id, uint64_t* req, const uint64_t* omega,
+                                             const uint64_t* twiddle, uint64_t n, uint64_t log_n,
+                                             uint64_t twiddle_size, const uint64_t* c) {
+  (void)twiddle;
+  (void)twiddle_size;
+  if (!size_ok(n, log_n)) return ZK_ERR_INVALID;
+  return (uint64_t)host_call(device_id, req, log_n, OP_COSET_FWD, omega, nullptr, c);
+}
+
+uint64_t parallel_msm_api(size_t device_id, const uint64_t* data, const void* bases, uint64_t length, void* acc) {
+  (void)device_id;
+  (void)data;
+  (void)bases;
+  (void)length;
+  (void)acc;
+  return ZK_ERR_UNSUPPORTED;
+}
+
+uint64_t zkgpu_coset_ifft_api(size_t device_id, uint64_t* req, const uint64_t* omega, uint64_t n, uint64_t log_n,
+                              const uint64_t* divisor, const uint64_t* c) {
+  if (!size_ok(n, log_n)) return ZK_ERR_INVALID;
+  return (uint64_t)host_call(device_id, req, log_n, OP_COSET_INV, omega, divisor, c);
+}
+
+int zkgpu_ntt_device(int device_id, const void* d_in, void* d_out, uint32_t log_n, int op, const uint64_t* omega,
+                     const uint64_t* divisor, const uint64_t* c, void* stream) {
+  if (!d_in || !d_out || !omega || log_n < 1 || log_n > MAX_LOG_N) return ZK_ERR_INVALID;
+  if (op < OP_FWD || op > OP_COSET_INV) return ZK_ERR_INVALID;
+  if ((op == OP_INV || op == OP_COSET_INV) && !divisor) return ZK_ERR_INVALID;
+  if ((op == OP_COSET_FWD || op == OP_COSET_INV) && !c) return ZK_ERR_INVALID;
+  int st = ZK_OK;
+  Engine* e = Engine::get(device_id, &st);
+  if (!e) return st;
+  JobSpec job;
+  job.in = d_in;
+  job.out = d_out;
+  job.log_n = log_n;
+  job.op = op;
+  job.omega = host::from_ptr(omega);
+  if (divisor) job.divisor = host::from_ptr(divisor);
+  if (c) job.coset = host::from_ptr(c);
+  return e->run(job, (hipStream_t)stream);
+}
+
+int zkgpu_domain_constant(uint32_t log_n, int which, uint64_t* out4) {
+  if (!out4 || log_n < 1 || log_n > MAX_LOG_N || which < 0 || which > 4) return ZK_ERR_INVALID;
+  const host::Fr v = std_constant(log_n, which);
+  memcpy(out4, v.w, 32);
+  return ZK_OK;
+}
+
+int zkgpu_ntt_std_device(int device_id, const void* d_in, void* d_out, uint32_t log_n, int op, void* stream) {
+  if (log_n < 1 || log_n > MAX_LOG_N) return ZK_ERR_INVALID;
+  const bool inverse = op == OP_INV || op == OP_COSET_INV;
+  const host::Fr omega = std_constant(log_n, inverse ? 1 : 0);
+  const host::Fr div = std_constant(log_n, 2);
+  const host::Fr c = std_constant(log_n, op == OP_COSET_INV ? 4 : 3);
+  return zkgpu_ntt_device(device_id, d_in, d_out, log_n, op, omega.w, div.w, c.w, stream);
+}
+
+int zkgpu_prepare(int device_id, uint32_t log_n, int op, const uint64_t* omega, const uint64_t* divisor,
+                  const uint64_t* c) {
+  if (!omega || log_n < 1 || log_n > MAX_LOG_N || op < OP_FWD || op > OP_COSET_INV) return ZK_ERR_INVALID;
+  int st = ZK_OK;
+  Engine* e = Engine::get(device_id, &st);
+  if (!e) return st;
+  JobSpec job;
+  job.log_n = log_n;
+  job.op = op;
+  job.omega = host::from_ptr(omega);
+  if (divisor) job.divisor = host::from_ptr(divisor);
+  if (c) job.coset = host::from_ptr(c);
+  JobTables t;
+  hipStream_t s = e->thread_stream();
+  st = e->prepare(job, t, s);
+  if (st) return st;
+  return hipStreamSynchronize(s) == hipSuccess ? ZK_OK : ZK_ERR_HIP;
+}
+
+int zkgpu_fill_random(int device_id, void* d_out, uint64_t count, uint64_t seed, uint64_t start, void* stream) {
+  if (!d_out) return ZK_ERR_INVALID;
+  int st = ZK_OK;
+  Engine* e = Engine::get(device_id, &st);
+  if (!e) return st;
+  if (count == 0) return ZK_OK;
+  return launch_fill_random(d_out, seed, start, count, (hipStream_t)stream) == hipSuccess ? ZK_OK : ZK_ERR_HIP;
+}
+
+int zkgpu_check_canonical(int device_id, const void* d_in, uint64_t count, uint32_t* d_flag, void* stream) {
+  if (!d_in || !d_flag) return ZK_ERR_INVALID;
+  int st = ZK_OK;
+  Engine* e = Engine::get(device_id, &st);
+  if (!e) return st;
+  if (count == 0) return ZK_OK;
+  return launch_check_canonical(d_in, count, d_flag, (hipStream_t)stream) == hipSuccess ? ZK_OK : ZK_ERR_HIP;
+}
+
+int zkgpu_plan_get(uint32_t log_n, uint32_t* digits, uint32_t* log_cols) {
+  if (log_n < 1 || log_n > MAX_LOG_N) return 0;
+  const Plan p = plan_for(log_n);
+  for (uint32_t i = 0; i < MAX_PASSES; ++i) {
+    if (digits) digits[i] = p.digit[i];
+    if (log_cols) log_cols[i] = p.log_c[i];
+  }
+  return (int)p.npass;
+}
+
+int zkgpu_plan_set(uint32_t log_n, uint32_t npass, const uint32_t* digits, const uint32_t* log_cols) {
+  if (log_n < 1 || log_n > MAX_LOG_N) return ZK_ERR_INVALID;
+  if (npass == 0) {
+    Plan d;
+    default_plan(log_n, d);
+    set_plan_override(d);
+    return ZK_OK;
+  }
+  if (npass > MAX_PASSES || !digits || !log_cols) return ZK_ERR_INVALID;
+  Plan p;
+  p.log_n = log_n;
+  p.npass = npass;
+  for (uint32_t i = 0; i < npass; ++i) {
+    p.digit[i] = digits[i];
+    p.log_c[i] = log_cols[i];
+  }
+  if (!plan_valid(p)) return ZK_ERR_INVALID;
+  set_plan_override(p);
+  return ZK_OK;
+}
+
+uint64_t zkgpu_table_bytes(int device_id) {
+  int st = ZK_OK;
+  Engine* e = Engine::get(device_id, &st);
+  return e ? (uint64_t)e->table_bytes() : 0;
+}
+
+int zkgpu_drop_tables(int device_id) {
+  int st = ZK_OK;
+  Engine* e = Engine::get(device_id, &st);
+  if (!e) return st;
+  e->drop_tables();
+  return ZK_OK;
+}
+
+const char* zkgpu_error_string(int code) {
+  switch (code) {
+    case ZK_OK: return "ok";
+    case ZK_ERR_INVALID: return "invalid argument";
+    case ZK_ERR_UNSUPPORTED: return "unsupported entry point";
+    case ZK_ERR_HIP: return "HIP runtime error";
+    case ZK_ERR_NOMEM: return "device allocation failed";
+    case ZK_ERR_NONCANONICAL: return "input element not below r";
+    case ZK_ERR_BUDGET: return "job exceeds the HBM budget";
+    case ZK_ERR_SHUTDOWN: return "server stopping";
+    default: return "unknown error";
+  }
+}
+
+int zkgpu_version(void) { return 100; }
+
+}  // extern "C"

@@ -1,0 +1,408 @@
+// Per-device NTT engine (see engine.hpp).
+#include "engine.hpp"
+
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+
+namespace zkgpu {
+
+// ---------------------------------------------------------------- plans
+
+static std::mutex g_plan_mu;
+static std::map<uint32_t, Plan> g_plan_override;
+
+bool plan_valid(const Plan& p) {
+  if (p.log_n < 1 || p.log_n > MAX_LOG_N || p.npass < 1 || p.npass > MAX_PASSES) return false;
+  uint32_t sum = 0;
+  for (uint32_t i = 0; i < p.npass; ++i) {
+    if (p.digit[i] < 1 || p.digit[i] > TILE_LOG_MAX) return false;
+    if (p.digit[i] + p.log_c[i] > TILE_LOG_MAX) return false;
+    sum += p.digit[i];
+  }
+  if (sum != p.log_n) return false;
+  if (p.npass == 1) return p.log_c[0] == 0;
+  uint32_t log_m = p.log_n;
+  for (uint32_t i = 0; i + 1 < p.npass; ++i) {
+    const uint32_t log_L = log_m - p.digit[i];  // columns of pass i
+    if (p.log_c[i] > log_L) return false;
+    log_m = log_L;
+  }
+  // last pass: its columns run over the first output digit
+  return p.log_c[p.npass - 1] <= p.digit[0];
+}
+
+bool default_plan(uint32_t log_n, Plan& p) {
+  p = Plan();
+  if (log_n < 1 || log_n > MAX_LOG_N) return false;
+  p.log_n = log_n;
+  if (log_n <= TILE_LOG_MAX) {
+    p.npass = 1;
+    p.digit[0] = log_n;
+    return true;
+  }
+  p.npass = log_n <= 2 * TILE_LOG_MAX ? 2 : 3;
+  uint32_t rem = log_n;
+  for (uint32_t i = 0; i < p.npass; ++i) {
+    const uint32_t left = p.npass - i;
+    p.digit[i] = (rem + left - 1) / left;
+    rem -= p.digit[i];
+  }
+  // columns per tile: tiles of at most 4096 elements, at least ~1024 tiles
+  // per pass when the size allows (256 CUs, several tiles each).
+  uint32_t log_m = log_n;
+  for (uint32_t i = 0; i < p.npass; ++i) {
+    const uint32_t avail = (i + 1 < p.npass) ? log_m - p.digit[i] : p.digit[0];
+    const uint32_t log_dfts = log_n - p.digit[i];
+    uint32_t c = log_dfts > 10 ? log_dfts - 10 : 0;
+    c = std::min(c, std::min(avail, TILE_LOG_MAX - p.digit[i]));
+    p.log_c[i] = c;
+    log_m -= p.digit[i];
+  }
+  return plan_valid(p);
+}
+
+Plan plan_for(uint32_t log_n) {
+  {
+    std::lock_guard<std::mutex> g(g_plan_mu);
+    auto it = g_plan_override.find(log_n);
+    if (it != g_plan_override.end()) return it->second;
+  }
+  Plan p;
+  default_plan(log_n, p);
+  return p;
+}
+void set_plan_override(const Plan& p) {
+  std::lock_guard<std::mutex> g(g_plan_mu);
+  g_plan_override[p.log_n] = p;
+}
+void clear_plan_overrides() {
+  std::lock_guard<std::mutex> g(g_plan_mu);
+  g_plan_override.clear();
+}
+
+// ---------------------------------------------------------------- engine
+
+DevBuf::~DevBuf() {
+  if (ptr) {
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    if (cur != device) (void)hipSetDevice(device);
+    (void)hipFree(ptr);
+    if (cur != device) (void)hipSetDevice(cur);
+  }
+}
+
+static std::mutex g_engines_mu;
+static std::map<int, std::unique_ptr<Engine>> g_engines;
+
+Engine* Engine::get(int device, int* status) {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) {
+    if (status) *status = ZK_ERR_INVALID;
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> g(g_engines_mu);
+  auto it = g_engines.find(device);
+  if (it == g_engines.end()) {
+    if (hipSetDevice(device) != hipSuccess || configure_kernels() != hipSuccess) {
+      if (status) *status = ZK_ERR_HIP;
+      return nullptr;
+    }
+    it = g_engines.emplace(device, std::unique_ptr<Engine>(new Engine(device))).first;
+  }
+  if (hipSetDevice(device) != hipSuccess) {
+    if (status) *status = ZK_ERR_HIP;
+    return nullptr;
+  }
+  if (status) *status = ZK_OK;
+  return it->second.get();
+}
+
+Engine::Engine(int device) : device_(device) {
+  size_t mb = 32768;  // resident table budget
+  if (const char* e = getenv("ZKGPU_TABLE_CACHE_MB")) mb = (size_t)strtoull(e, nullptr, 10);
+  table_cap_ = mb << 20;
+}
+
+std::string Engine::last_error() {
+  std::lock_guard<std::mutex> g(mu_);
+  return last_error_;
+}
+void Engine::set_error(const std::string& e) {
+  std::lock_guard<std::mutex> g(mu_);
+  last_error_ = e;
+}
+
+std::shared_ptr<DevBuf> Engine::alloc(size_t bytes) {
+  auto b = std::make_shared<DevBuf>();
+  b->device = device_;
+  b->bytes = bytes;
+  if (hipMalloc(&b->ptr, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    b->ptr = nullptr;
+    return nullptr;
+  }
+  return b;
+}
+
+hipStream_t Engine::thread_stream() {
+  thread_local std::map<int, hipStream_t> streams;
+  auto it = streams.find(device_);
+  if (it != streams.end()) return it->second;
+  hipStream_t s = nullptr;
+  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return nullptr;
+  streams[device_] = s;
+  return s;
+}
+
+void* Engine::workspace(hipStream_t s, size_t bytes, int slot) {
+  std::lock_guard<std::mutex> g(mu_);
+  auto key = std::make_pair(s, slot);
+  auto it = ws_.find(key);
+  if (it != ws_.end() && it->second->bytes >= bytes) return it->second->ptr;
+  if (it != ws_.end()) {
+    // the old buffer may still be in use by work queued on s
+    (void)hipStreamSynchronize(s);
+    ws_.erase(it);
+  }
+  auto b = alloc(bytes);
+  if (!b) return nullptr;
+  ws_[key] = b;
+  return b->ptr;
+}
+
+std::shared_ptr<DevBuf> Engine::table(const std::string& key, size_t bytes, bool* fresh) {
+  std::lock_guard<std::mutex> g(mu_);
+  auto it = tables_.find(key);
+  if (it != tables_.end()) {
+    lru_.remove(key);
+    lru_.push_front(key);
+    *fresh = false;
+    return it->second;
+  }
+  // evict least recently used tables nobody else holds
+  while (table_bytes_ + bytes > table_cap_ && !lru_.empty()) {
+    bool evicted = false;
+    for (auto r = lru_.rbegin(); r != lru_.rend(); ++r) {
+      auto t = tables_.find(*r);
+      if (t != tables_.end() && t->second.use_count() == 1) {
+        (void)hipDeviceSynchronize();  // work on other streams may still read it
+        table_bytes_ -= t->second->bytes;
+        tables_.erase(t);
+        lru_.erase(std::next(r).base());
+        evicted = true;
+        break;
+      }
+    }
+    if (!evicted) break;
+  }
+  auto b = alloc(bytes);
+  if (!b) return nullptr;
+  tables_[key] = b;
+  lru_.push_front(key);
+  table_bytes_ += bytes;
+  *fresh = true;
+  return b;
+}
+
+size_t Engine::table_bytes() {
+  std::lock_guard<std::mutex> g(mu_);
+  return table_bytes_;
+}
+void Engine::drop_tables() {
+  (void)hipDeviceSynchronize();
+  std::lock_guard<std::mutex> g(mu_);
+  tables_.clear();
+  lru_.clear();
+  table_bytes_ = 0;
+}
+
+static std::string hex_fr(const host::Fr& a) {
+  char buf[80];
+  snprintf(buf, sizeof(buf), "%016llx%016llx%016llx%016llx", (unsigned long long)a.w[3],
+           (unsigned long long)a.w[2], (unsigned long long)a.w[1], (unsigned long long)a.w[0]);
+  return buf;
+}
+
+// lo[x] = base^(x*step), x < 2^h; hi[y] = base^(y*step*2^h), y < 2^(log_count-h); 48-byte limbs.
+int Engine::build_pow_lohi(const host::Fr& base, uint64_t step, uint32_t log_count, uint32_t h,
+                           std::shared_ptr<DevBuf>& lo, std::shared_ptr<DevBuf>& hi, hipStream_t s) {
+  uint32_t w[8];
+  host::to_words32(w, base);
+  const uint32_t nlo = 1u << h, nhi = 1u << (log_count - h);
+  lo = alloc((size_t)nlo * 48);
+  hi = alloc((size_t)nhi * 48);
+  if (!lo || !hi) return ZK_ERR_NOMEM;
+  if (launch_pow_table(lo->ptr, w, step, nlo, true, s) != hipSuccess) return ZK_ERR_HIP;
+  if (launch_pow_table(hi->ptr, w, step << h, nhi, true, s) != hipSuccess) return ZK_ERR_HIP;
+  return ZK_OK;
+}
+
+int Engine::prepare(const JobSpec& job, JobTables& t, hipStream_t s) {
+  std::lock_guard<std::mutex> build_lock(build_mu_);
+  t = JobTables();
+  t.plan = plan_for(job.log_n);
+  if (!plan_valid(t.plan)) return ZK_ERR_INVALID;
+  const Plan& p = t.plan;
+  const uint32_t k = job.log_n;
+  const std::string om = hex_fr(job.omega);
+  std::vector<std::shared_ptr<DevBuf>> temps;  // kept alive until the build is synchronised
+  bool built = false;
+
+  // roots of the tile DFTs: omega_Nt^e, Nt = 2^min(k, 12)
+  {
+    const uint32_t lnt = std::min(k, TILE_LOG_MAX);
+    bool fresh = false;
+    t.roots = table("R" + std::to_string(k) + om, (size_t)48 << lnt, &fresh);
+    if (!t.roots) return ZK_ERR_NOMEM;
+    if (fresh) {
+      uint32_t w[8];
+      host::to_words32(w, job.omega);
+      if (launch_pow_table(t.roots->ptr, w, 1ull << (k - lnt), 1u << lnt, true, s) != hipSuccess) return ZK_ERR_HIP;
+      built = true;
+    }
+  }
+  // inter-pass twiddles W_p[t * L + i] = omega_m^(t * i)
+  uint32_t log_m = k;
+  for (uint32_t i = 0; i + 1 < p.npass; ++i) {
+    const uint32_t log_L = log_m - p.digit[i];
+    bool fresh = false;
+    t.w[i] = table("W" + std::to_string(k) + "." + std::to_string(log_m) + "." + std::to_string(log_L) + om,
+                   (size_t)32 << log_m, &fresh);
+    if (!t.w[i]) return ZK_ERR_NOMEM;
+    if (fresh) {
+      const uint32_t h = (log_m + 1) / 2;
+      std::shared_ptr<DevBuf> lo, hi;
+      int st = build_pow_lohi(job.omega, 1ull << (k - log_m), log_m, h, lo, hi, s);
+      if (st) return st;
+      if (launch_twiddle_table(t.w[i]->ptr, lo->ptr, hi->ptr, h, log_m, log_L, s) != hipSuccess) return ZK_ERR_HIP;
+      temps.push_back(lo);
+      temps.push_back(hi);
+      built = true;
+    }
+    log_m = log_L;
+  }
+  // coset powers
+  if (job.op == OP_COSET_FWD || job.op == OP_COSET_INV) {
+    const bool post = job.op == OP_COSET_INV;
+    std::string key = std::string(post ? "Q" : "P") + std::to_string(k) + hex_fr(job.coset);
+    if (post) key += hex_fr(job.divisor);
+    bool fresh = false;
+    auto tab = table(key, (size_t)32 << k, &fresh);
+    if (!tab) return ZK_ERR_NOMEM;
+    if (fresh) {
+      const uint32_t h = (k + 1) / 2;
+      std::shared_ptr<DevBuf> lo, hi;
+      int st = build_pow_lohi(job.coset, 1, k, h, lo, hi, s);
+      if (st) return st;
+      uint32_t sc[8];
+      host::to_words32(sc, job.divisor);
+      if (launch_pow_table_2level(tab->ptr, lo->ptr, hi->ptr, h, 1ull << k, post ? sc : nullptr, s) != hipSuccess)
+        return ZK_ERR_HIP;
+      temps.push_back(lo);
+      temps.push_back(hi);
+      built = true;
+    }
+    (post ? t.post : t.pre) = tab;
+  }
+  if (job.op == OP_INV) {
+    t.uscale = true;
+    host::to_limbs261(t.scale261, job.divisor);
+  }
+  if (built) {
+    // publish only finished tables: other streams may use them right away
+    if (hipStreamSynchronize(s) != hipSuccess) return ZK_ERR_HIP;
+  }
+  return ZK_OK;
+}
+
+int Engine::build_descs(const JobSpec& job, const JobTables& t, void* ws, PassDesc* descs, uint32_t* ndesc) const {
+  const Plan& p = t.plan;
+  const uint32_t k = job.log_n;
+  const uint32_t lnt = std::min(k, TILE_LOG_MAX);
+  if (p.npass > 1 && !ws) return ZK_ERR_INVALID;
+  uint32_t log_m = k;
+  for (uint32_t i = 0; i < p.npass; ++i) {
+    PassDesc d;
+    memset(&d, 0, sizeof(d));
+    const uint32_t N = p.digit[i], c = p.log_c[i];
+    d.roots = t.roots->ptr;
+    d.root_log = lnt;
+    d.log_n_tile = N;
+    d.log_c = c;
+    const bool last = i + 1 == p.npass;
+    if (!last) {
+      const uint32_t log_L = log_m - N;
+      d.in = i == 0 ? job.in : ws;
+      d.out = ws;
+      d.wtab = t.w[i]->ptr;
+      d.pre = (i == 0 && t.pre) ? t.pre->ptr : nullptr;
+      d.log_n_lo = log_L - c;
+      d.num_tiles = 1u << (k - N - c);
+      d.in_lo = d.out_lo = d.w_lo = 1u << c;
+      d.in_hi = d.out_hi = 1u << log_m;
+      d.w_hi = 0;
+      d.in_ts = d.out_ts = d.w_ts = 1u << log_L;
+      d.in_cs = d.out_cs = d.w_cs = 1;
+      log_m = log_L;
+    } else if (p.npass == 1) {
+      d.in = job.in;
+      d.out = job.out;
+      d.pre = t.pre ? t.pre->ptr : nullptr;
+      d.log_n_lo = 0;
+      d.num_tiles = 1;
+      d.in_ts = d.out_ts = 1;
+    } else {
+      // columns over the first output digit k_1 (C consecutive values), one
+      // tile row per value of the middle digits
+      const uint32_t n1 = p.digit[0];
+      const uint32_t log_hi = k - n1 - N;  // middle digits (0 or the single middle digit)
+      d.in = ws;
+      d.out = job.out;
+      d.log_n_lo = n1 - c;
+      d.num_tiles = 1u << (n1 - c + log_hi);
+      d.in_cs = 1u << (k - n1);
+      d.in_lo = 1u << (k - n1 + c);
+      d.in_hi = 1u << N;
+      d.in_ts = 1;
+      d.out_cs = 1;
+      d.out_lo = 1u << c;
+      d.out_hi = 1u << n1;
+      d.out_ts = 1u << (k - N);
+    }
+    if (last) {
+      d.post = t.post ? t.post->ptr : nullptr;
+      d.flags = PASS_CANON | (t.uscale ? PASS_USCALE : 0u);
+      if (t.uscale) memcpy(d.scale, t.scale261, sizeof(t.scale261));
+    }
+    descs[i] = d;
+  }
+  *ndesc = p.npass;
+  return ZK_OK;
+}
+
+int Engine::run(const JobSpec& job, hipStream_t s) {
+  if (job.log_n < 1 || job.log_n > MAX_LOG_N || !job.in || !job.out) return ZK_ERR_INVALID;
+  if (job.op < OP_FWD || job.op > OP_COSET_INV) return ZK_ERR_INVALID;
+  JobTables t;
+  int st = prepare(job, t, s);
+  if (st) return st;
+  void* ws = nullptr;
+  if (t.plan.npass > 1) {
+    ws = workspace(s, (size_t)32 << job.log_n, 0);
+    if (!ws) return ZK_ERR_NOMEM;
+  }
+  PassDesc descs[MAX_PASSES];
+  uint32_t nd = 0;
+  st = build_descs(job, t, ws, descs, &nd);
+  if (st) return st;
+  for (uint32_t i = 0; i < nd; ++i) {
+    if (launch_pass(descs[i], s) != hipSuccess) return ZK_ERR_HIP;
+  }
+  return ZK_OK;
+}
+
+}  // namespace zkgpu
