@@ -1,0 +1,192 @@
+"""Benchmark of the BN256 Fr NTT on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+Workload (the metric's k=20 configuration): one step = forward, inverse
+(x n^-1) and coset-forward (g = 7) transforms of 2^20 uniform random field
+elements, device resident (inputs already in HBM when the timed region
+starts). The same line also reports the 2^24 forward transform.
+
+Prints ONE JSON line. `value` is elements transformed per second over the
+timed steps. `roofline` describes the dominant kernel (the NTT pass kernel)
+against the binding bound, the VALU integer multiply rate: algorithmic work
+is (n/2) log2 n Montgomery products per transform (+ n per scaling), each
+136 32x32->64 multiply-adds (8-word CIOS). `roofline_hbm` gives the same
+kernel against HBM bandwidth (64 bytes per element per pass).
+`cpu_baseline` times the CPU restatement of halo2's best_fft (oracle/) on
+the host cores over a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+PEAK_HBM_GBS = 8000.0
+# 256 CUs x 64 lanes x 2.4 GHz; v_mad_u64_u32 issues at the full VALU rate on gfx950
+PEAK_MAD_TOPS = 256 * 64 * 2.4e9 / 1e12
+MADS_PER_MODMUL = 136
+
+
+def modmuls(log_n, op):
+    n = 1 << log_n
+    m = (n // 2) * log_n
+    if op != "fwd":
+        m += n  # divisor or coset scaling
+    if op == "coset_inv":
+        m += n
+    return m
+
+
+def passes(Z, log_n):
+    npass, digits, cols = Z.plan_get(log_n)
+    return npass, digits, cols
+
+
+def cpu_baseline(budget_s=15.0):
+    """halo2 best_fft restated in C (oracle/), timed on the host cores."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle import oracle as O  # the checker, used here only as the timed CPU path
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, 16))
+    k = 20
+    x = O.gen(2001, 1 << k)
+    ops = ("fwd", "inv", "coset_fwd")
+    O.run_op("fwd", x, k, threads=cores)  # warm-up
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        for op in ops:
+            O.run_op(op, x, k, threads=cores)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or steps >= 20:
+            break
+    value = steps * len(ops) * (1 << k) / el
+    return {"value": value, "unit": "elem/s", "cores": cores, "kind": "port",
+            "sample": f"{steps} steps of fwd+inv+coset_fwd at 2^20 (halo2 best_fft restated in C, "
+                      f"log_split=floor(log2 {cores}))"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-k24", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (--gpus {args.gpus} requested); "
+              "the transform's multi-GPU split is design-only", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import zk_gpu_amd as Z
+
+    dev = torch.device("cuda", 0)
+    stream = torch.cuda.current_stream(dev)
+    sptr = stream.cuda_stream
+    k = 20
+    n = 1 << k
+    x = torch.empty(n * 4, dtype=torch.int64, device=dev)
+    outs = [torch.empty(n * 4, dtype=torch.int64, device=dev) for _ in range(3)]
+    Z.fill_random(x.data_ptr(), n, 2001, stream=sptr)
+    ops = [("fwd", Z.OP_FWD), ("inv", Z.OP_INV), ("coset_fwd", Z.OP_COSET_FWD)]
+    for _, op in ops:
+        Z.prepare_std(k, op)
+
+    def step():
+        for (_, op), o in zip(ops, outs):
+            Z.ntt_std_device(x.data_ptr(), o.data_ptr(), k, op, stream=sptr)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t_wall = time.perf_counter()
+    e0.record(stream)
+    for _ in range(args.steps):
+        step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_wall
+    ms = e0.elapsed_time(e1) / args.steps
+    value = 3 * n / (ms * 1e-3)
+
+    npass, digits, cols = passes(Z, k)
+    launches = 3 * npass
+    launch_ms = ms / launches
+    mm_step = sum(modmuls(k, name) for name, _ in ops)
+    mads_per_launch = mm_step * MADS_PER_MODMUL / launches
+    achieved = mads_per_launch / (launch_ms * 1e-3) / 1e12
+    bytes_per_launch = 64 * n
+    hbm = bytes_per_launch / (launch_ms * 1e-3) / 1e9
+    kernel_name = f"ntt_pass_kernel<{digits[0]}>" if len(set(digits)) == 1 else \
+        "ntt_pass_kernel<" + ",".join(str(d) for d in digits) + ">"
+
+    res = {
+        "metric": "BN256 Fr NTT elem/s at k=20/24; served mixed-size vs plain; p50/p99 lat",
+        "value": value,
+        "unit": "elem/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic (seeded SplitMix64 uniform Fr elements, Montgomery form, device resident)",
+        "config": {"workload": "fwd+inv+coset_fwd NTT over BN256 Fr at n=2^20 (one step = 3 transforms)",
+                   "log_n": k, "ops": [o for o, _ in ops], "coset_g": 7,
+                   "plan": {"digits": digits, "log_cols": cols}},
+        "roofline": {"bound": "valu", "kernel": kernel_name, "achieved": achieved, "peak": PEAK_MAD_TOPS,
+                     "unit": "Tmad/s", "frac": achieved / PEAK_MAD_TOPS, "traffic": None,
+                     "launch_ms": launch_ms, "launches_per_step": launches,
+                     "work_per_launch": f"{mads_per_launch:.4g} 32x32->64 multiply-adds "
+                                        f"({mm_step / launches:.4g} Montgomery products x 136)"},
+        "roofline_hbm": {"bound": "hbm", "achieved": hbm, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                         "frac": hbm / PEAK_HBM_GBS, "bytes_per_launch": bytes_per_launch},
+        "wall_s_timed_region": wall,
+    }
+
+    if not args.no_k24:
+        k24 = 24
+        n24 = 1 << k24
+        x24 = torch.empty(n24 * 4, dtype=torch.int64, device=dev)
+        y24 = torch.empty(n24 * 4, dtype=torch.int64, device=dev)
+        Z.fill_random(x24.data_ptr(), n24, 3001, stream=sptr)
+        Z.prepare_std(k24, Z.OP_FWD)
+        for _ in range(2):
+            Z.ntt_std_device(x24.data_ptr(), y24.data_ptr(), k24, Z.OP_FWD, stream=sptr)
+        torch.cuda.synchronize()
+        reps = max(3, args.steps // 2)
+        e0.record(stream)
+        for _ in range(reps):
+            Z.ntt_std_device(x24.data_ptr(), y24.data_ptr(), k24, Z.OP_FWD, stream=sptr)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms24 = e0.elapsed_time(e1) / reps
+        np24, d24, c24 = passes(Z, k24)
+        l_ms = ms24 / np24
+        mads = modmuls(k24, "fwd") * MADS_PER_MODMUL / np24
+        ach = mads / (l_ms * 1e-3) / 1e12
+        res["k24"] = {"op": "fwd", "ms": ms24, "elem_per_s": n24 / (ms24 * 1e-3),
+                      "plan": {"digits": d24, "log_cols": c24},
+                      "roofline": {"bound": "valu", "achieved": ach, "peak": PEAK_MAD_TOPS, "unit": "Tmad/s",
+                                   "frac": ach / PEAK_MAD_TOPS, "launch_ms": l_ms},
+                      "roofline_hbm": {"bound": "hbm", "achieved": 64 * n24 / (l_ms * 1e-3) / 1e9,
+                                       "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                                       "frac": 64 * n24 / (l_ms * 1e-3) / 1e9 / PEAK_HBM_GBS}}
+        del x24, y24
+
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

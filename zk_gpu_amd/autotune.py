@@ -1,0 +1,150 @@
+"""Autotuner for the pass split of the NTT (radix/tile/pass schedule).
+
+For each transform size 2^k it times every candidate split -- number of
+passes, the log2 DFT length of each pass (the tile radix) and the log2
+columns per tile (the tile width) -- on the device and keeps the fastest.
+Results are written as plain text lines
+
+    log_n npass d0 d1 d2 c0 c1 c2 ms
+
+to zk_gpu_amd/plans_gfx950.txt, which the library reads at start-up
+(ZKGPU_PLAN_FILE overrides the path).
+
+    python -m zk_gpu_amd.autotune --k 12-24 [--reps 5] [--out PATH]
+"""
+import argparse
+import itertools
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DEFAULT_OUT = os.path.join(HERE, "plans_gfx950.txt")
+TILE_LOG_MAX = 12
+
+
+def candidates(k, min_tiles_log=8, max_passes=3):
+    """Valid pass splits of 2^k: digits each <= 12, tile N*C <= 4096,
+    column limits as in the engine (pass p < P: columns <= 2^(remaining),
+    last pass: columns <= 2^digit0)."""
+    out = []
+    for npass in range(1, max_passes + 1):
+        if npass == 1 and k > TILE_LOG_MAX:
+            continue
+        for digits in itertools.product(range(1, TILE_LOG_MAX + 1), repeat=npass):
+            if sum(digits) != k:
+                continue
+            # balanced splits only: the modmul count barely depends on the
+            # split, while lopsided splits lose tile parallelism
+            if npass == 2 and abs(digits[0] - digits[1]) > 4:
+                continue
+            if npass == 3 and max(digits) - min(digits) > 3:
+                continue
+            # tile width: one common target tile size for every pass
+            # (2^t elements per tile, t in 8..12) or no columns at all
+            avails = []
+            log_m = k
+            for i, d in enumerate(digits):
+                if npass == 1:
+                    avail = 0
+                elif i + 1 < npass:
+                    avail = log_m - d
+                else:
+                    avail = digits[0]
+                avails.append(min(avail, TILE_LOG_MAX - d))
+                log_m -= d
+            seen = set()
+            for target in (0, 8, 9, 10, 11, 12):
+                cols = tuple(min(max(target - d, 0), a) for d, a in zip(digits, avails))
+                if cols in seen:
+                    continue
+                seen.add(cols)
+                # enough tiles per pass to fill the chip (unless the size is small)
+                if all((k - d - c) >= min(min_tiles_log, k - d) for d, c in zip(digits, cols)):
+                    out.append((list(digits), list(cols)))
+    return out
+
+
+def time_plan(Z, torch, k, digits, cols, x, y, stream, reps):
+    Z.plan_set(k, digits, cols)
+    Z.lib().zkgpu_drop_tables(0)
+    Z.prepare_std(k, Z.OP_FWD)
+    Z.ntt_std_device(x.data_ptr(), y.data_ptr(), k, Z.OP_FWD, stream=stream.cuda_stream)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        Z.ntt_std_device(x.data_ptr(), y.data_ptr(), k, Z.OP_FWD, stream=stream.cuda_stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def tune(ks, reps=5, out=DEFAULT_OUT, verbose=True, max_candidates=None):
+    import torch
+
+    import zk_gpu_amd as Z
+    dev = torch.device("cuda", 0)
+    stream = torch.cuda.current_stream(dev)
+    best = {}
+    for k in ks:
+        n = 1 << k
+        x = torch.empty(n * 4, dtype=torch.int64, device=dev)
+        y = torch.empty_like(x)
+        Z.fill_random(x.data_ptr(), n, 99, stream=stream.cuda_stream)
+        cands = candidates(k)
+        if max_candidates:
+            cands = cands[:max_candidates]
+        results = []
+        for digits, cols in cands:
+            ms = time_plan(Z, torch, k, digits, cols, x, y, stream, reps)
+            results.append((ms, digits, cols))
+            if verbose:
+                print(f"k={k} digits={digits} cols={cols} {ms:.4f} ms", flush=True)
+        results.sort()
+        best[k] = results[0]
+        Z.plan_reset(k)
+        if verbose:
+            print(f"BEST k={k}: {results[0]}", flush=True)
+        del x, y
+        Z.lib().zkgpu_drop_tables(0)
+    if out:
+        existing = {}
+        if os.path.exists(out):
+            for line in open(out):
+                f = line.split()
+                if len(f) >= 9 and not line.startswith("#"):
+                    existing[int(f[0])] = line.rstrip("\n")
+        for k, (ms, digits, cols) in best.items():
+            d = digits + [0] * (3 - len(digits))
+            c = cols + [0] * (3 - len(cols))
+            existing[k] = f"{k} {len(digits)} {d[0]} {d[1]} {d[2]} {c[0]} {c[1]} {c[2]} {ms:.5f}"
+        with open(out, "w") as f:
+            f.write("# log_n npass d0 d1 d2 c0 c1 c2 ms  (autotuned on gfx950, forward transform)\n")
+            for k in sorted(existing):
+                f.write(existing[k] + "\n")
+    return best
+
+
+def _parse_ks(s):
+    ks = []
+    for part in s.split(","):
+        if "-" in part:
+            a, b = part.split("-")
+            ks.extend(range(int(a), int(b) + 1))
+        else:
+            ks.append(int(part))
+    return ks
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--k", default="12-24")
+    ap.add_argument("--reps", type=int, default=5)
+    ap.add_argument("--out", default=DEFAULT_OUT)
+    ap.add_argument("--max-candidates", type=int, default=None)
+    a = ap.parse_args()
+    sys.path.insert(0, os.path.dirname(HERE))
+    t = time.time()
+    tune(_parse_ks(a.k), reps=a.reps, out=a.out, max_candidates=a.max_candidates)
+    print(f"autotune took {time.time() - t:.1f} s")

@@ -1,6 +1,7 @@
 // Per-device NTT engine (see engine.hpp).
 #include "engine.hpp"
 
+#include <dlfcn.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -64,7 +65,47 @@ bool default_plan(uint32_t log_n, Plan& p) {
   return plan_valid(p);
 }
 
+// Tuned splits: lines "log_n npass d0 d1 d2 c0 c1 c2 [ms]" in the file named by
+// ZKGPU_PLAN_FILE, else plans_gfx950.txt next to this library.
+static void load_plan_file() {
+  std::string path;
+  if (const char* e = getenv("ZKGPU_PLAN_FILE")) {
+    path = e;
+  } else {
+    Dl_info info;
+    if (dladdr((void*)&load_plan_file, &info) && info.dli_fname) {
+      path = info.dli_fname;
+      const size_t slash = path.rfind('/');
+      path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) + "/plans_gfx950.txt";
+    }
+  }
+  if (path.empty()) return;
+  FILE* f = fopen(path.c_str(), "r");
+  if (!f) return;
+  char line[256];
+  while (fgets(line, sizeof(line), f)) {
+    if (line[0] == '#') continue;
+    unsigned v[8];
+    if (sscanf(line, "%u %u %u %u %u %u %u %u", &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6], &v[7]) != 8)
+      continue;
+    Plan p;
+    p.log_n = v[0];
+    p.npass = v[1];
+    for (int i = 0; i < 3; ++i) {
+      p.digit[i] = v[2 + i];
+      p.log_c[i] = v[5 + i];
+    }
+    if (plan_valid(p)) g_plan_override[p.log_n] = p;
+  }
+  fclose(f);
+}
+
 Plan plan_for(uint32_t log_n) {
+  static std::once_flag once;
+  std::call_once(once, [] {
+    std::lock_guard<std::mutex> g(g_plan_mu);
+    load_plan_file();
+  });
   {
     std::lock_guard<std::mutex> g(g_plan_mu);
     auto it = g_plan_override.find(log_n);
@@ -75,6 +116,7 @@ Plan plan_for(uint32_t log_n) {
   return p;
 }
 void set_plan_override(const Plan& p) {
+  (void)plan_for(p.log_n);  // load the plan file first so it cannot overwrite this override
   std::lock_guard<std::mutex> g(g_plan_mu);
   g_plan_override[p.log_n] = p;
 }
