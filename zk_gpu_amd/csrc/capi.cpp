@@ -48,14 +48,25 @@ int host_call(size_t device_id, uint64_t* req, uint64_t log_n, int op, const uin
   return ZK_OK;
 }
 
-host::Fr std_constant(uint32_t log_n, int which) {
-  switch (which) {
-    case 0: return host::root_of_unity(log_n);
-    case 1: return host::inv(host::root_of_unity(log_n));
-    case 2: return host::inv(host::from_u64(1ull << log_n));
-    case 3: return host::from_u64(7);
-    default: return host::inv(host::from_u64(7));
+// Standard-domain constants (omega, omega^-1, n^-1, g, g^-1) for every
+// size, derived once: each needs field inversions on the host.
+struct StdDomain {
+  host::Fr c[MAX_LOG_N + 1][5];
+  StdDomain() {
+    const host::Fr g = host::from_u64(7), ginv = host::inv(g);
+    for (uint32_t k = 1; k <= MAX_LOG_N; ++k) {
+      c[k][0] = host::root_of_unity(k);
+      c[k][1] = host::inv(c[k][0]);
+      c[k][2] = host::inv(host::from_u64(1ull << k));
+      c[k][3] = g;
+      c[k][4] = ginv;
+    }
   }
+};
+
+const host::Fr& std_constant(uint32_t log_n, int which) {
+  static const StdDomain dom;
+  return dom.c[log_n][which];
 }
 
 }  // namespace
@@ -145,17 +156,16 @@ int zkgpu_ntt_device(int device_id, const void* d_in, void* d_out, uint32_t log_
 
 int zkgpu_domain_constant(uint32_t log_n, int which, uint64_t* out4) {
   if (!out4 || log_n < 1 || log_n > MAX_LOG_N || which < 0 || which > 4) return ZK_ERR_INVALID;
-  const host::Fr v = std_constant(log_n, which);
-  memcpy(out4, v.w, 32);
+  memcpy(out4, std_constant(log_n, which).w, 32);
   return ZK_OK;
 }
 
 int zkgpu_ntt_std_device(int device_id, const void* d_in, void* d_out, uint32_t log_n, int op, void* stream) {
   if (log_n < 1 || log_n > MAX_LOG_N) return ZK_ERR_INVALID;
   const bool inverse = op == OP_INV || op == OP_COSET_INV;
-  const host::Fr omega = std_constant(log_n, inverse ? 1 : 0);
-  const host::Fr div = std_constant(log_n, 2);
-  const host::Fr c = std_constant(log_n, op == OP_COSET_INV ? 4 : 3);
+  const host::Fr& omega = std_constant(log_n, inverse ? 1 : 0);
+  const host::Fr& div = std_constant(log_n, 2);
+  const host::Fr& c = std_constant(log_n, op == OP_COSET_INV ? 4 : 3);
   return zkgpu_ntt_device(device_id, d_in, d_out, log_n, op, omega.w, div.w, c.w, stream);
 }
 

@@ -141,26 +141,37 @@ static std::mutex g_engines_mu;
 static std::map<int, std::unique_ptr<Engine>> g_engines;
 
 Engine* Engine::get(int device, int* status) {
-  int count = 0;
-  if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) {
+  static const int count = [] {
+    int c = 0;
+    return hipGetDeviceCount(&c) == hipSuccess ? c : 0;
+  }();
+  if (device < 0 || device >= count) {
     if (status) *status = ZK_ERR_INVALID;
     return nullptr;
   }
-  std::lock_guard<std::mutex> g(g_engines_mu);
-  auto it = g_engines.find(device);
-  if (it == g_engines.end()) {
-    if (hipSetDevice(device) != hipSuccess || configure_kernels() != hipSuccess) {
+  Engine* e = nullptr;
+  {
+    std::lock_guard<std::mutex> g(g_engines_mu);
+    auto it = g_engines.find(device);
+    if (it == g_engines.end()) {
+      if (hipSetDevice(device) != hipSuccess || configure_kernels() != hipSuccess) {
+        if (status) *status = ZK_ERR_HIP;
+        return nullptr;
+      }
+      it = g_engines.emplace(device, std::unique_ptr<Engine>(new Engine(device))).first;
+    }
+    e = it->second.get();
+  }
+  // the device is per host thread; switch only when needed
+  int cur = -1;
+  if (hipGetDevice(&cur) != hipSuccess || cur != device) {
+    if (hipSetDevice(device) != hipSuccess) {
       if (status) *status = ZK_ERR_HIP;
       return nullptr;
     }
-    it = g_engines.emplace(device, std::unique_ptr<Engine>(new Engine(device))).first;
-  }
-  if (hipSetDevice(device) != hipSuccess) {
-    if (status) *status = ZK_ERR_HIP;
-    return nullptr;
   }
   if (status) *status = ZK_OK;
-  return it->second.get();
+  return e;
 }
 
 Engine::Engine(int device) : device_(device) {
@@ -216,7 +227,7 @@ void* Engine::workspace(hipStream_t s, size_t bytes, int slot) {
   return b->ptr;
 }
 
-std::shared_ptr<DevBuf> Engine::table(const std::string& key, size_t bytes, bool* fresh) {
+std::shared_ptr<DevBuf> Engine::table(const TableKey& key, size_t bytes, bool* fresh) {
   std::lock_guard<std::mutex> g(mu_);
   auto it = tables_.find(key);
   if (it != tables_.end()) {
@@ -262,11 +273,18 @@ void Engine::drop_tables() {
   table_bytes_ = 0;
 }
 
-static std::string hex_fr(const host::Fr& a) {
-  char buf[80];
-  snprintf(buf, sizeof(buf), "%016llx%016llx%016llx%016llx", (unsigned long long)a.w[3],
-           (unsigned long long)a.w[2], (unsigned long long)a.w[1], (unsigned long long)a.w[0]);
-  return buf;
+static TableKey make_key(char kind, uint32_t a, uint32_t b, uint32_t c, const host::Fr& x,
+                         const host::Fr* y = nullptr) {
+  TableKey k;
+  k.kind = kind;
+  k.a = a;
+  k.b = b;
+  k.c = c;
+  for (int i = 0; i < 4; ++i) {
+    k.e[i] = x.w[i];
+    k.e[4 + i] = y ? y->w[i] : 0;
+  }
+  return k;
 }
 
 // lo[x] = base^(x*step), x < 2^h; hi[y] = base^(y*step*2^h), y < 2^(log_count-h); 48-byte limbs.
@@ -290,7 +308,6 @@ int Engine::prepare(const JobSpec& job, JobTables& t, hipStream_t s) {
   if (!plan_valid(t.plan)) return ZK_ERR_INVALID;
   const Plan& p = t.plan;
   const uint32_t k = job.log_n;
-  const std::string om = hex_fr(job.omega);
   std::vector<std::shared_ptr<DevBuf>> temps;  // kept alive until the build is synchronised
   bool built = false;
 
@@ -298,7 +315,7 @@ int Engine::prepare(const JobSpec& job, JobTables& t, hipStream_t s) {
   {
     const uint32_t lnt = std::min(k, TILE_LOG_MAX);
     bool fresh = false;
-    t.roots = table("R" + std::to_string(k) + om, (size_t)48 << lnt, &fresh);
+    t.roots = table(make_key('R', k, lnt, 0, job.omega), (size_t)48 << lnt, &fresh);
     if (!t.roots) return ZK_ERR_NOMEM;
     if (fresh) {
       uint32_t w[8];
@@ -312,8 +329,7 @@ int Engine::prepare(const JobSpec& job, JobTables& t, hipStream_t s) {
   for (uint32_t i = 0; i + 1 < p.npass; ++i) {
     const uint32_t log_L = log_m - p.digit[i];
     bool fresh = false;
-    t.w[i] = table("W" + std::to_string(k) + "." + std::to_string(log_m) + "." + std::to_string(log_L) + om,
-                   (size_t)32 << log_m, &fresh);
+    t.w[i] = table(make_key('W', k, log_m, log_L, job.omega), (size_t)32 << log_m, &fresh);
     if (!t.w[i]) return ZK_ERR_NOMEM;
     if (fresh) {
       const uint32_t h = (log_m + 1) / 2;
@@ -330,8 +346,7 @@ int Engine::prepare(const JobSpec& job, JobTables& t, hipStream_t s) {
   // coset powers
   if (job.op == OP_COSET_FWD || job.op == OP_COSET_INV) {
     const bool post = job.op == OP_COSET_INV;
-    std::string key = std::string(post ? "Q" : "P") + std::to_string(k) + hex_fr(job.coset);
-    if (post) key += hex_fr(job.divisor);
+    const TableKey key = make_key(post ? 'Q' : 'P', k, 0, 0, job.coset, post ? &job.divisor : nullptr);
     bool fresh = false;
     auto tab = table(key, (size_t)32 << k, &fresh);
     if (!tab) return ZK_ERR_NOMEM;
@@ -366,34 +381,45 @@ int Engine::build_descs(const JobSpec& job, const JobTables& t, void* ws, PassDe
   const uint32_t k = job.log_n;
   const uint32_t lnt = std::min(k, TILE_LOG_MAX);
   if (p.npass > 1 && !ws) return ZK_ERR_INVALID;
-  uint32_t log_m = k;
+  const uint32_t n = 1u << k;
+  uint32_t log_m = k, prev_log_m = k;
   for (uint32_t i = 0; i < p.npass; ++i) {
     PassDesc d;
     memset(&d, 0, sizeof(d));
     const uint32_t N = p.digit[i], c = p.log_c[i];
+    const bool last = i + 1 == p.npass;
     d.roots = t.roots->ptr;
     d.root_log = lnt;
     d.log_n_tile = N;
     d.log_c = c;
-    const bool last = i + 1 == p.npass;
+    // input: the caller's packed data in the first pass, the planar
+    // workspace after it; the previous pass's twiddles are applied on load
+    if (i == 0) {
+      d.in = job.in;
+      d.pre = t.pre ? t.pre->ptr : nullptr;
+      d.pre_mask = n - 1;
+    } else {
+      d.in = ws;
+      d.in_plane = n;
+      d.flags |= PASS_IN_PLANAR;
+      d.pre = t.w[i - 1]->ptr;
+      d.pre_mask = (1u << prev_log_m) - 1u;
+    }
     if (!last) {
       const uint32_t log_L = log_m - N;
-      d.in = i == 0 ? job.in : ws;
       d.out = ws;
-      d.wtab = t.w[i]->ptr;
-      d.pre = (i == 0 && t.pre) ? t.pre->ptr : nullptr;
+      d.out_plane = n;
+      d.flags |= PASS_OUT_PLANAR;
       d.log_n_lo = log_L - c;
       d.num_tiles = 1u << (k - N - c);
-      d.in_lo = d.out_lo = d.w_lo = 1u << c;
+      d.in_lo = d.out_lo = 1u << c;
       d.in_hi = d.out_hi = 1u << log_m;
-      d.w_hi = 0;
-      d.in_ts = d.out_ts = d.w_ts = 1u << log_L;
-      d.in_cs = d.out_cs = d.w_cs = 1;
+      d.in_ts = d.out_ts = 1u << log_L;
+      d.in_cs = d.out_cs = 1;
+      prev_log_m = log_m;
       log_m = log_L;
     } else if (p.npass == 1) {
-      d.in = job.in;
       d.out = job.out;
-      d.pre = t.pre ? t.pre->ptr : nullptr;
       d.log_n_lo = 0;
       d.num_tiles = 1;
       d.in_ts = d.out_ts = 1;
@@ -402,7 +428,6 @@ int Engine::build_descs(const JobSpec& job, const JobTables& t, void* ws, PassDe
       // tile row per value of the middle digits
       const uint32_t n1 = p.digit[0];
       const uint32_t log_hi = k - n1 - N;  // middle digits (0 or the single middle digit)
-      d.in = ws;
       d.out = job.out;
       d.log_n_lo = n1 - c;
       d.num_tiles = 1u << (n1 - c + log_hi);
@@ -417,7 +442,7 @@ int Engine::build_descs(const JobSpec& job, const JobTables& t, void* ws, PassDe
     }
     if (last) {
       d.post = t.post ? t.post->ptr : nullptr;
-      d.flags = PASS_CANON | (t.uscale ? PASS_USCALE : 0u);
+      d.flags |= PASS_CANON | (t.uscale ? PASS_USCALE : 0u);
       if (t.uscale) memcpy(d.scale, t.scale261, sizeof(t.scale261));
     }
     descs[i] = d;
@@ -434,7 +459,7 @@ int Engine::run(const JobSpec& job, hipStream_t s) {
   if (st) return st;
   void* ws = nullptr;
   if (t.plan.npass > 1) {
-    ws = workspace(s, (size_t)32 << job.log_n, 0);
+    ws = workspace(s, (size_t)36 << job.log_n, 0);  // planar limb form
     if (!ws) return ZK_ERR_NOMEM;
   }
   PassDesc descs[MAX_PASSES];

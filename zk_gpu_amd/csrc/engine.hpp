@@ -67,6 +67,24 @@ struct JobSpec {
   host::Fr coset{};    // COSET_FWD: a[i] *= coset^i first; COSET_INV: out[i] *= coset^i last
 };
 
+// Resident-table key: kind ('R' roots, 'W' twiddles, 'P' coset pre-scale,
+// 'Q' coset post-scale), sizes, and the field elements the table derives from.
+struct TableKey {
+  char kind = 0;
+  uint32_t a = 0, b = 0, c = 0;
+  uint64_t e[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  bool operator<(const TableKey& o) const {
+    if (kind != o.kind) return kind < o.kind;
+    if (a != o.a) return a < o.a;
+    if (b != o.b) return b < o.b;
+    if (c != o.c) return c < o.c;
+    for (int i = 0; i < 8; ++i)
+      if (e[i] != o.e[i]) return e[i] < o.e[i];
+    return false;
+  }
+  bool operator==(const TableKey& o) const { return !(*this < o) && !(o < *this); }
+};
+
 // Owning device allocation.
 struct DevBuf {
   void* ptr = nullptr;
@@ -95,8 +113,8 @@ class Engine {
   // Fetch or build every table a job needs (stream-ordered on `s`, then
   // synchronised once so that the tables are safe for every stream).
   int prepare(const JobSpec& job, JobTables& t, hipStream_t s);
-  // Pass descriptors of a prepared job; ws must hold 2^log_n elements when
-  // the plan has more than one pass.
+  // Pass descriptors of a prepared job; ws must hold 36 * 2^log_n bytes (planar
+  // limb form) when the plan has more than one pass.
   int build_descs(const JobSpec& job, const JobTables& t, void* ws, PassDesc* descs, uint32_t* ndesc) const;
   // Enqueue one transform on `s` (asynchronous).
   int run(const JobSpec& job, hipStream_t s);
@@ -112,7 +130,7 @@ class Engine {
 
  private:
   explicit Engine(int device);
-  std::shared_ptr<DevBuf> table(const std::string& key, size_t bytes, bool* fresh);
+  std::shared_ptr<DevBuf> table(const TableKey& key, size_t bytes, bool* fresh);
   std::shared_ptr<DevBuf> alloc(size_t bytes);
   int build_pow_lohi(const host::Fr& base, uint64_t step, uint32_t log_count, uint32_t h,
                      std::shared_ptr<DevBuf>& lo, std::shared_ptr<DevBuf>& hi, hipStream_t s);
@@ -120,8 +138,8 @@ class Engine {
   int device_;
   std::mutex build_mu_;  // table lookups and builds: a table is visible only once built
   std::mutex mu_;
-  std::map<std::string, std::shared_ptr<DevBuf>> tables_;
-  std::list<std::string> lru_;
+  std::map<TableKey, std::shared_ptr<DevBuf>> tables_;
+  std::list<TableKey> lru_;
   size_t table_bytes_ = 0;
   size_t table_cap_ = 0;
   std::map<std::pair<hipStream_t, int>, std::shared_ptr<DevBuf>> ws_;

@@ -55,16 +55,26 @@ __device__ __forceinline__ void store_limbs48(void* base, uint64_t idx, const fe
   p[1] = make_uint4(v.l[4], v.l[5], v.l[6], v.l[7]);
   p[2] = make_uint4(v.l[8], 0u, 0u, 0u);
 }
-// LDS: nine planes of `plane` dwords
-__device__ __forceinline__ fe lds_load(const uint32_t* lds, uint32_t idx, uint32_t plane) {
+// Planar limb form (workspace between passes and LDS): limbs 0-3 in a uint4
+// plane, limbs 4-7 in a second uint4 plane, limb 8 in a u32 plane.
+__device__ __forceinline__ fe load_planar(const void* base, uint32_t plane, uint32_t idx) {
+  const uint4* a = (const uint4*)base;
+  const uint4* b = a + plane;
+  const uint32_t* c = (const uint32_t*)(b + plane);
+  const uint4 x = a[idx], y = b[idx];
   fe r;
-#pragma unroll
-  for (int q = 0; q < NLIMBS; ++q) r.l[q] = lds[q * plane + idx];
+  r.l[0] = x.x; r.l[1] = x.y; r.l[2] = x.z; r.l[3] = x.w;
+  r.l[4] = y.x; r.l[5] = y.y; r.l[6] = y.z; r.l[7] = y.w;
+  r.l[8] = c[idx];
   return r;
 }
-__device__ __forceinline__ void lds_store(uint32_t* lds, uint32_t idx, uint32_t plane, const fe& v) {
-#pragma unroll
-  for (int q = 0; q < NLIMBS; ++q) lds[q * plane + idx] = v.l[q];
+__device__ __forceinline__ void store_planar(void* base, uint32_t plane, uint32_t idx, const fe& v) {
+  uint4* a = (uint4*)base;
+  uint4* b = a + plane;
+  uint32_t* c = (uint32_t*)(b + plane);
+  a[idx] = make_uint4(v.l[0], v.l[1], v.l[2], v.l[3]);
+  b[idx] = make_uint4(v.l[4], v.l[5], v.l[6], v.l[7]);
+  c[idx] = v.l[8];
 }
 
 template <int R>
@@ -88,56 +98,154 @@ __host__ __device__ constexpr int bitrev_c(int v, int bits) {
 
 // ---------------------------------------------------------------- DFT
 
-// Value bounds (units of r) of the radix-2 DIF network: inputs < 3r; stage s
-// outputs are sums (<= 2B), multiplier outputs (< 1.5r) or, for the unit
-// twiddle, differences x + C_s r - y (<= 2B + 2). B: 3, 8, 18, 38, 78.
-// C_s = B_s + 2 makes C_s r exceed every y of stage s, as sub<C> requires.
-template <int S> struct StageC { static constexpr uint32_t value = 2 * StageC<S - 1>::value; };
-template <> struct StageC<0> { static constexpr uint32_t value = 5; };
+// Bound state of one register: lo = max of limbs 0..7, top = max of limb 8,
+// v = value bound in units of r (value < v * r), norm = limbs 0..7 < 2^29.
+struct Bnd {
+  uint64_t lo, top;
+  uint32_t v;
+  bool norm;
+};
+constexpr uint64_t LIMB_MAX32 = 0xFFFFFFFFull;
+constexpr uint64_t MUL_IN_MAX = MUL_A_LIMB_MAX;   // multiplier operand limb bound
+constexpr uint32_t MUL_IN_VMAX = 160;             // keeps multiplier outputs below 2r
+constexpr uint64_t TOP_PER_R = 0x30644full;       // r < 0x30644f * 2^232
+constexpr uint64_t SUBK_LO = (1ull << 30);        // sub<C> adds K[i] < 2^30 to limbs 0..7
 
+constexpr Bnd bnd_normalized(uint32_t v) { return Bnd{LIMB_MASK, v * TOP_PER_R, v, true}; }
+constexpr bool mul_ok(const Bnd& b) { return b.lo <= MUL_IN_MAX && b.top <= MUL_IN_MAX && b.v <= MUL_IN_VMAX; }
+
+// Normalisation schedule of a radix-2 DIF network on R registers, decided at
+// compile time from the bounds: an operand is normalised (or, when its value
+// grows too large, value-reduced) only when an operation would otherwise
+// break a contract of fr29.hpp (limbs < 2^32, multiplier operands <=
+// MUL_A_LIMB_MAX, sub<C> needs a normalized subtrahend with V < (C-1) r).
+// Inputs are normalized with V < 6r; outputs satisfy the multiplier contract.
+template <int R>
+struct DifSchedule {
+  static constexpr int LR = Log2<R>::value;
+  static constexpr int NB = LR * R / 2;
+  // per butterfly: 0 nothing, 1 normalize, 2 reduce_value
+  uint8_t fx[NB > 0 ? NB : 1], fy[NB > 0 ? NB : 1];
+  uint32_t c[NB > 0 ? NB : 1];
+  uint8_t fout[R];
+};
+
+template <int R>
+constexpr DifSchedule<R> make_dif_schedule() {
+  DifSchedule<R> S{};
+  Bnd b[R] = {};
+  for (int i = 0; i < R; ++i) b[i] = bnd_normalized(6);
+  int idx = 0;
+  for (int h = R / 2; h >= 1; h /= 2) {
+    for (int j = 0; j < h; ++j) {
+      for (int blk = 0; blk < R; blk += 2 * h) {
+        const int x = blk + j, y = blk + j + h;
+        uint8_t fx = 0, fy = 0;
+        Bnd bx = b[x], by = b[y];
+        auto apply = [](Bnd& t, uint8_t f) {
+          if (f == 2) t = bnd_normalized(4);
+          else if (f == 1) t = bnd_normalized(t.v);
+        };
+        auto fits = [&](const Bnd& X, const Bnd& Y) {
+          const uint32_t C = Y.v + 1;
+          const uint64_t slo = X.lo + Y.lo, stop = X.top + Y.top;
+          const uint64_t dlo = X.lo + SUBK_LO, dtop = X.top + C * TOP_PER_R;
+          const uint32_t sv = X.v + Y.v, dv = X.v + C;
+          bool ok = Y.norm && slo <= LIMB_MAX32 && stop <= LIMB_MAX32 && sv <= MUL_IN_VMAX && dv <= MUL_IN_VMAX;
+          if (j != 0) ok = ok && dlo <= MUL_IN_MAX && dtop <= MUL_IN_MAX;
+          else ok = ok && dlo <= LIMB_MAX32 && dtop <= LIMB_MAX32;
+          return ok;
+        };
+        // escalate: normalize y, then x, then reduce both
+        if (!by.norm) { fy = 1; apply(by, 1); }
+        if (!fits(bx, by) && !bx.norm) { fx = 1; apply(bx, 1); }
+        if (!fits(bx, by)) { fx = 2; apply(bx, 2); }
+        if (!fits(bx, by)) { fy = 2; apply(by, 2); }
+        const uint32_t C = by.v + 1;
+        S.fx[idx] = fx;
+        S.fy[idx] = fy;
+        S.c[idx] = C;
+        Bnd s{bx.lo + by.lo, bx.top + by.top, bx.v + by.v, false};
+        Bnd d{};
+        if (j != 0) d = bnd_normalized(2);
+        else d = Bnd{bx.lo + SUBK_LO, bx.top + C * TOP_PER_R, bx.v + C, false};
+        b[x] = s;
+        b[y] = d;
+        ++idx;
+      }
+    }
+  }
+  for (int i = 0; i < R; ++i) {
+    S.fout[i] = mul_ok(b[i]) ? 0 : (b[i].v > MUL_IN_VMAX ? 2 : 1);
+  }
+  return S;
+}
+
+template <int R>
+struct DifSched {
+  static constexpr DifSchedule<R> value = make_dif_schedule<R>();
+};
+
+template <uint8_t F>
+__device__ __forceinline__ void fix_bound(fe& a) {
+  if constexpr (F == 1) normalize(a);
+  else if constexpr (F == 2) reduce_value(a);
+}
+
+// the sub<C> constants are instantiated per butterfly; C is at most MUL_IN_VMAX + 1
 template <int S, int R, int H>
 __device__ __forceinline__ void dif_stage(fe* u, const uint4* roots, uint32_t root_log) {
   constexpr int LR = Log2<R>::value;
-  constexpr uint32_t C = StageC<S>::value;
+  constexpr int BASE = S * (R / 2);
   static_for<H>([&](auto jc) {
     constexpr int j = decltype(jc)::value;
     fe w;
     if constexpr (j != 0) w = load_limbs48(roots, (uint64_t)(j * (R / (2 * H))) << (root_log - LR));
     static_for<R / (2 * H)>([&](auto bc) {
-      constexpr int b = decltype(bc)::value * 2 * H;
+      constexpr int bi = decltype(bc)::value;
+      constexpr int b = bi * 2 * H;
+      constexpr int idx = BASE + j * (R / (2 * H)) + bi;
+      constexpr uint8_t FX = DifSched<R>::value.fx[idx];
+      constexpr uint8_t FY = DifSched<R>::value.fy[idx];
+      constexpr uint32_t C = DifSched<R>::value.c[idx];
+      fix_bound<FX>(u[b + j]);
+      fix_bound<FY>(u[b + j + H]);
       const fe x = u[b + j], y = u[b + j + H];
-      fe s = add(x, y);
-      normalize(s);
-      fe d = sub<C>(x, y);
-      if constexpr (j == 0) normalize(d);
-      else d = mont_mul(d, w);
-      u[b + j] = s;
-      u[b + j + H] = d;
+      u[b + j] = add(x, y);
+      if constexpr (j == 0) u[b + j + H] = sub<C>(x, y);
+      else u[b + j + H] = mont_mul(sub<C>(x, y), w);
     });
   });
 }
 
 // Length-R DFT in registers: natural order in, bit-reversed order out.
-// omega_R^e = roots[e << (root_log - log R)]. Inputs normalized, V < 3r;
-// outputs normalized, V <= 78r.
+// omega_R^e = roots[e << (root_log - log R)]. Inputs normalized with V < 6r;
+// outputs satisfy the multiplier's operand contract.
 template <int R>
 __device__ __forceinline__ void dft_regs(fe* u, const uint4* roots, uint32_t root_log) {
-  if constexpr (R >= 16) dif_stage<0, R, 8>(u, roots, root_log);
-  if constexpr (R >= 8) dif_stage<(R >= 16 ? 1 : 0), R, 4>(u, roots, root_log);
-  if constexpr (R >= 4) dif_stage<(R >= 16 ? 2 : R >= 8 ? 1 : 0), R, 2>(u, roots, root_log);
-  dif_stage<(R >= 16 ? 3 : R >= 8 ? 2 : R >= 4 ? 1 : 0), R, 1>(u, roots, root_log);
+  constexpr int LR = Log2<R>::value;
+  static_for<LR>([&](auto sc) {
+    constexpr int s = decltype(sc)::value;
+    dif_stage<s, R, (R >> (s + 1))>(u, roots, root_log);
+  });
+  static_for<R>([&](auto ic) {
+    constexpr int i = decltype(ic)::value;
+    fix_bound<DifSched<R>::value.fout[i]>(u[i]);
+  });
 }
 
 // ---------------------------------------------------------------- tile
 
 struct TileCtx {
   uint32_t logN, logC, c, i;
-  uint32_t in_base, out_base, w_base;
-  uint32_t plane;  // N * C dwords per LDS limb plane
+  uint32_t in_base, out_base;
+  uint32_t plane;  // N * C elements per LDS limb plane
   bool active;
 };
 
-template <int E, int R>
+// PREFETCH: load the round twiddles before the LDS reads and the barrier
+// (more registers live across the barrier; used where that fits in 256 VGPRs).
+template <int E, int R, bool PREFETCH>
 __device__ __forceinline__ void ntt_round(fe (&u)[E], const PassDesc& d, const TileCtx& t, uint32_t logp,
                                           bool first, bool last, uint32_t* lds) {
   constexpr int S = E / R;
@@ -147,39 +255,63 @@ __device__ __forceinline__ void ntt_round(fe (&u)[E], const PassDesc& d, const T
   const uint32_t NR = 1u << (t.logN - LR);
   const uint32_t pmask = (1u << logp) - 1u;
   const uint4* roots = (const uint4*)d.roots;
+  const bool in_planar = d.flags & PASS_IN_PLANAR;
 
-  if (t.active) {
-    static_for<S>([&](auto sc) {
-      constexpr int s = decltype(sc)::value;
-      const uint32_t iv = t.i + s * NE;
-      static_for<R>([&](auto jc) {
-        constexpr int j = decltype(jc)::value;
-        const uint32_t pos = iv + j * NR;
-        if (first) {
-          const uint32_t off = t.in_base + pos * d.in_ts;
-          fe v = load_packed(d.in, off);
-          if (d.pre) v = mont_mul(v, load_packed(d.pre, off));
-          u[s * R + j] = v;
-        } else {
-          u[s * R + j] = lds_load(lds, t.c + (pos << t.logC), t.plane);
-        }
-      });
-    });
-  }
-  if (!first) __syncthreads();  // every LDS read of this round precedes any overwrite
-  if (t.active) {
-    if (logp > 0) {
-      const uint32_t sh = d.root_log - (logp + LR);
+  if (first) {
+    if (t.active) {
       static_for<S>([&](auto sc) {
         constexpr int s = decltype(sc)::value;
-        const uint32_t k = (t.i + s * NE) & pmask;
-        reduce_value(u[s * R]);
-        static_for<R, 1>([&](auto jc) {
+        const uint32_t iv = t.i + s * NE;
+        static_for<R>([&](auto jc) {
           constexpr int j = decltype(jc)::value;
-          u[s * R + j] = mont_mul(u[s * R + j], load_limbs48(roots, (uint64_t)(j * k) << sh));
+          const uint32_t off = t.in_base + (iv + j * NR) * d.in_ts;
+          fe v = in_planar ? load_planar(d.in, d.in_plane, off) : load_packed(d.in, off);
+          if (d.pre) v = mont_mul(v, load_packed(d.pre, off & d.pre_mask));
+          else if (in_planar) reduce_value(v);
+          u[s * R + j] = v;
         });
       });
     }
+  } else {
+    // round twiddles omega_{pR}^{jk}: issued ahead of the LDS reads and the
+    // barrier so that their latency overlaps them
+    fe tw[S][R];
+    const uint32_t sh = d.root_log - (logp + LR);
+    auto load_tw = [&] {
+      static_for<S>([&](auto sc) {
+        constexpr int s = decltype(sc)::value;
+        const uint32_t k = (t.i + s * NE) & pmask;
+        static_for<R, 1>([&](auto jc) {
+          constexpr int j = decltype(jc)::value;
+          tw[s][j] = load_limbs48(roots, (uint64_t)(j * k) << sh);
+        });
+      });
+    };
+    if (t.active) {
+      if constexpr (PREFETCH) load_tw();
+      static_for<S>([&](auto sc) {
+        constexpr int s = decltype(sc)::value;
+        const uint32_t iv = t.i + s * NE;
+        static_for<R>([&](auto jc) {
+          constexpr int j = decltype(jc)::value;
+          u[s * R + j] = load_planar(lds, t.plane, t.c + ((iv + j * NR) << t.logC));
+        });
+      });
+    }
+    __syncthreads();  // every LDS read of this round precedes any overwrite
+    if (t.active) {
+      if constexpr (!PREFETCH) load_tw();
+      static_for<S>([&](auto sc) {
+        constexpr int s = decltype(sc)::value;
+        reduce_value(u[s * R]);
+        static_for<R, 1>([&](auto jc) {
+          constexpr int j = decltype(jc)::value;
+          u[s * R + j] = mont_mul(u[s * R + j], tw[s][j]);
+        });
+      });
+    }
+  }
+  if (t.active) {
     static_for<S>([&](auto sc) { dft_regs<R>(&u[decltype(sc)::value * R], roots, d.root_log); });
     static_for<S>([&](auto sc) {
       constexpr int s = decltype(sc)::value;
@@ -191,25 +323,30 @@ __device__ __forceinline__ void ntt_round(fe (&u)[E], const PassDesc& d, const T
         const uint32_t pos = base + ((uint32_t)m << logp);
         fe v = u[s * R + bitrev_c(m, LR)];
         if (last) {
-          if (d.wtab) v = mont_mul(v, load_packed(d.wtab, t.w_base + pos * d.w_ts));
           const uint32_t off = t.out_base + pos * d.out_ts;
-          if (d.post) v = mont_mul(v, load_packed(d.post, off));
-          if (d.flags & PASS_USCALE) {
-            fe sc;
-#pragma unroll
-            for (int q = 0; q < NLIMBS; ++q) sc.l[q] = d.scale[q];
-            v = mont_mul(v, sc);
-          }
-          uint32_t w[8];
-          if (d.flags & PASS_CANON) {
-            to_canonical_words(w, v);
+          if (d.flags & PASS_OUT_PLANAR) {
+            // DFT outputs meet the multiplier's operand contract; the next
+            // pass multiplies them by its pre-twiddle on load
+            store_planar(d.out, d.out_plane, off, v);
           } else {
-            reduce_value(v);
-            pack_words(w, v);
+            if (d.post) v = mont_mul(v, load_packed(d.post, off));
+            if (d.flags & PASS_USCALE) {
+              fe sc;
+#pragma unroll
+              for (int q = 0; q < NLIMBS; ++q) sc.l[q] = d.scale[q];
+              v = mont_mul(v, sc);
+            }
+            uint32_t w[8];
+            if (d.flags & PASS_CANON) {
+              to_canonical_words(w, v);
+            } else {
+              reduce_value(v);
+              pack_words(w, v);
+            }
+            store_words(d.out, off, w);
           }
-          store_words(d.out, off, w);
         } else {
-          lds_store(lds, t.c + (pos << t.logC), t.plane, v);
+          store_planar(lds, t.plane, t.c + (pos << t.logC), v);
         }
       });
     });
@@ -224,6 +361,7 @@ template <int LOGN> struct TileShape {
   static constexpr int FULL = LOGN / LE;                 // rounds of radix E
   static constexpr int REM = LOGN - FULL * LE;           // log2 radix of the last round (0: none)
   static constexpr int ROUNDS = FULL + (REM ? 1 : 0);
+  static constexpr bool PREFETCH = LOGN <= 9;  // larger tiles would spill at 2 waves/SIMD
 };
 
 template <int LOGN, int RIDX>
@@ -233,7 +371,7 @@ __device__ __forceinline__ void ntt_rounds(fe (&u)[TileShape<LOGN>::E], const Pa
   if constexpr (RIDX < Sh::ROUNDS) {
     constexpr int LR = RIDX < Sh::FULL ? Sh::LE : Sh::REM;
     constexpr uint32_t LOGP = (uint32_t)(RIDX * Sh::LE);
-    ntt_round<Sh::E, (1 << LR)>(u, d, t, LOGP, RIDX == 0, RIDX == Sh::ROUNDS - 1, lds);
+    ntt_round<Sh::E, (1 << LR), Sh::PREFETCH>(u, d, t, LOGP, RIDX == 0, RIDX == Sh::ROUNDS - 1, lds);
     ntt_rounds<LOGN, RIDX + 1>(u, d, t, lds);
   }
 }
@@ -253,7 +391,6 @@ __device__ __forceinline__ void ntt_tile(const PassDesc& d, uint32_t tile, uint3
   const uint32_t tlo = tile & ((1u << d.log_n_lo) - 1u), thi = tile >> d.log_n_lo;
   t.in_base = tlo * d.in_lo + thi * d.in_hi + t.c * d.in_cs;
   t.out_base = tlo * d.out_lo + thi * d.out_hi + t.c * d.out_cs;
-  t.w_base = tlo * d.w_lo + thi * d.w_hi + t.c * d.w_cs;
   fe u[Sh::E];
   ntt_rounds<LOGN, 0>(u, d, t, lds);
 }
