@@ -120,6 +120,46 @@ int zkgpu_plan_get(uint32_t log_n, uint32_t* digits, uint32_t* log_cols);
  * ZKGPU_ERR_INVALID for an inconsistent split. npass = 0 restores the default. */
 int zkgpu_plan_set(uint32_t log_n, uint32_t npass, const uint32_t* digits, const uint32_t* log_cols);
 
+/* Same for the two plan tables: kind 0 = single transforms (tiles up to 2^12
+ * elements), kind 1 = batched transforms (tiles up to 2^11 elements). */
+int zkgpu_plan_get_kind(uint32_t log_n, int kind, uint32_t* digits, uint32_t* log_cols);
+int zkgpu_plan_set_kind(uint32_t log_n, int kind, uint32_t npass, const uint32_t* digits, const uint32_t* log_cols);
+
+/* ---------------------------------------------------------------------
+ * Batched transforms (no counterpart in the reference): many device-resident
+ * polynomials of uneven sizes and mixed ops run as one persistent-kernel
+ * launch per pass level, the levels captured once in a hipGraph and replayed
+ * by every zkgpu_batch_launch. Each job's result is bit-identical to the
+ * same job run alone.
+ * --------------------------------------------------------------------- */
+
+#define ZKGPU_JOB_STD_DOMAIN 1u /* ignore omega/divisor/coset: standard domain of size 2^log_n, g = 7 */
+
+typedef struct zkgpu_job {
+  const void* d_in;     /* device pointer, 2^log_n elements */
+  void* d_out;          /* device pointer, may equal d_in */
+  uint32_t log_n;
+  int32_t op;           /* ZKGPU_OP_* */
+  uint32_t flags;       /* ZKGPU_JOB_* */
+  uint32_t reserved;
+  uint64_t omega[4];    /* as zkgpu_ntt_device */
+  uint64_t divisor[4];
+  uint64_t coset[4];
+} zkgpu_job;
+
+/* Plan a batch, build (or fetch) its resident tables and workspaces and
+ * capture its launch graph. The job array is copied; the device buffers it
+ * names must stay valid until the batch is destroyed. */
+int zkgpu_batch_create(int device_id, const zkgpu_job* jobs, uint32_t count, uint64_t* handle);
+/* Enqueue one run of the batch on `stream` (asynchronous). */
+int zkgpu_batch_launch(uint64_t handle, void* stream);
+/* Pass levels, total tiles and total elements of a batch. */
+int zkgpu_batch_info(uint64_t handle, uint32_t* levels, uint32_t* tiles, uint64_t* elements);
+/* Wait for the device and free the batch. */
+int zkgpu_batch_destroy(uint64_t handle);
+/* create + launch + synchronize + destroy. */
+int zkgpu_ntt_batch_device(int device_id, const zkgpu_job* jobs, uint32_t count, void* stream);
+
 /* Resident table bytes on a device; drop them with zkgpu_drop_tables. */
 uint64_t zkgpu_table_bytes(int device_id);
 int zkgpu_drop_tables(int device_id);

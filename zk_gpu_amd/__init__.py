@@ -55,6 +55,13 @@ SIGNATURES = {
     "zkgpu_check_canonical": (_I, [_I, _P, _U64, _P, _P]),
     "zkgpu_plan_get": (_I, [_U32, _P, _P]),
     "zkgpu_plan_set": (_I, [_U32, _U32, _P, _P]),
+    "zkgpu_plan_get_kind": (_I, [_U32, _I, _P, _P]),
+    "zkgpu_plan_set_kind": (_I, [_U32, _I, _U32, _P, _P]),
+    "zkgpu_batch_create": (_I, [_I, _P, _U32, _P]),
+    "zkgpu_batch_launch": (_I, [_U64, _P]),
+    "zkgpu_batch_info": (_I, [_U64, _P, _P, _P]),
+    "zkgpu_batch_destroy": (_I, [_U64]),
+    "zkgpu_ntt_batch_device": (_I, [_I, _P, _U32, _P]),
     "zkgpu_table_bytes": (_U64, [_I]),
     "zkgpu_drop_tables": (_I, [_I]),
     "zkgpu_error_string": (ctypes.c_char_p, [_I]),
@@ -178,22 +185,82 @@ def fill_random(d_out, count, seed, start=0, stream=None, device_id=0):
     check(lib().zkgpu_fill_random(device_id, d_out, count, seed, start, stream), "zkgpu_fill_random")
 
 
-def plan_get(log_n):
+PLAN_SINGLE = 0
+PLAN_BATCH = 1
+
+
+def plan_get(log_n, kind=PLAN_SINGLE):
     d = np.zeros(3, dtype=np.uint32)
     c = np.zeros(3, dtype=np.uint32)
-    npass = lib().zkgpu_plan_get(log_n, d.ctypes.data_as(ctypes.c_void_p), c.ctypes.data_as(ctypes.c_void_p))
+    npass = lib().zkgpu_plan_get_kind(log_n, kind, d.ctypes.data_as(ctypes.c_void_p),
+                                      c.ctypes.data_as(ctypes.c_void_p))
     return int(npass), [int(x) for x in d[:npass]], [int(x) for x in c[:npass]]
 
 
-def plan_set(log_n, digits, log_cols):
+def plan_set(log_n, digits, log_cols, kind=PLAN_SINGLE):
     d = np.array(list(digits) + [0] * (3 - len(digits)), dtype=np.uint32)
     c = np.array(list(log_cols) + [0] * (3 - len(log_cols)), dtype=np.uint32)
-    check(lib().zkgpu_plan_set(log_n, len(digits), d.ctypes.data_as(ctypes.c_void_p),
-                               c.ctypes.data_as(ctypes.c_void_p)), "zkgpu_plan_set")
+    check(lib().zkgpu_plan_set_kind(log_n, kind, len(digits), d.ctypes.data_as(ctypes.c_void_p),
+                                    c.ctypes.data_as(ctypes.c_void_p)), "zkgpu_plan_set")
 
 
-def plan_reset(log_n):
-    check(lib().zkgpu_plan_set(log_n, 0, None, None), "zkgpu_plan_set")
+def plan_reset(log_n, kind=PLAN_SINGLE):
+    check(lib().zkgpu_plan_set_kind(log_n, kind, 0, None, None), "zkgpu_plan_set")
+
+
+# ---------------------------------------------------------------- batches
+
+JOB_STD_DOMAIN = 1
+
+
+class Job(ctypes.Structure):
+    """zkgpu_job (include/zkgpu_ntt.h)."""
+    _fields_ = [("d_in", ctypes.c_void_p), ("d_out", ctypes.c_void_p), ("log_n", ctypes.c_uint32),
+                ("op", ctypes.c_int32), ("flags", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("omega", ctypes.c_uint64 * 4), ("divisor", ctypes.c_uint64 * 4), ("coset", ctypes.c_uint64 * 4)]
+
+
+def make_job(d_in, d_out, log_n, op, omega=None, divisor=None, c=None):
+    """A job on device pointers; without constants it uses the standard domain (g = 7)."""
+    j = Job()
+    j.d_in, j.d_out, j.log_n, j.op = d_in, d_out, log_n, op
+    if omega is None:
+        j.flags = JOB_STD_DOMAIN
+    else:
+        for name, v in (("omega", omega), ("divisor", divisor), ("coset", c)):
+            if v is not None:
+                getattr(j, name)[:] = [int(x) for x in _elem(v)]
+    return j
+
+
+class Batch:
+    """Many device-resident transforms as one captured launch graph."""
+
+    def __init__(self, jobs, device_id=0):
+        arr = (Job * len(jobs))(*jobs)
+        self._jobs = arr
+        h = ctypes.c_uint64(0)
+        check(lib().zkgpu_batch_create(device_id, arr, len(jobs), ctypes.byref(h)), "zkgpu_batch_create")
+        self.handle = h.value
+
+    def launch(self, stream=None):
+        check(lib().zkgpu_batch_launch(self.handle, stream), "zkgpu_batch_launch")
+
+    def info(self):
+        lv, tl, el = ctypes.c_uint32(0), ctypes.c_uint32(0), ctypes.c_uint64(0)
+        check(lib().zkgpu_batch_info(self.handle, ctypes.byref(lv), ctypes.byref(tl), ctypes.byref(el)))
+        return {"levels": lv.value, "tiles": tl.value, "elements": el.value}
+
+    def close(self):
+        if self.handle:
+            lib().zkgpu_batch_destroy(self.handle)
+            self.handle = 0
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def version():

@@ -7,10 +7,13 @@ Results are written as plain text lines
 
     log_n npass d0 d1 d2 c0 c1 c2 ms
 
-to zk_gpu_amd/plans_gfx950.txt, which the library reads at start-up
-(ZKGPU_PLAN_FILE overrides the path).
+to zk_gpu_amd/plans_gfx950.txt (single transforms, tiles up to 2^12
+elements) or zk_gpu_amd/plans_batch_gfx950.txt (batched transforms, tiles up
+to 2^11 elements, timed as a batch of same-size jobs of about 2^22 elements
+in total), which the library reads at start-up (ZKGPU_PLAN_FILE /
+ZKGPU_BATCH_PLAN_FILE override the paths).
 
-    python -m zk_gpu_amd.autotune --k 12-24 [--reps 5] [--out PATH]
+    python -m zk_gpu_amd.autotune --k 12-24 [--kind single|batch] [--reps 5] [--out PATH]
 """
 import argparse
 import itertools
@@ -19,15 +22,17 @@ import sys
 import time
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-DEFAULT_OUT = os.path.join(HERE, "plans_gfx950.txt")
-TILE_LOG_MAX = 12
+DEFAULT_OUT = {"single": os.path.join(HERE, "plans_gfx950.txt"),
+               "batch": os.path.join(HERE, "plans_batch_gfx950.txt")}
+TILE_LOG = {"single": 12, "batch": 11}
 
 
-def candidates(k, min_tiles_log=8, max_passes=3):
+def candidates(k, min_tiles_log=8, max_passes=3, tile_log_max=12):
     """Valid pass splits of 2^k: digits each <= 12, tile N*C <= 4096,
     column limits as in the engine (pass p < P: columns <= 2^(remaining),
     last pass: columns <= 2^digit0)."""
     out = []
+    TILE_LOG_MAX = tile_log_max
     for npass in range(1, max_passes + 1):
         if npass == 1 and k > TILE_LOG_MAX:
             continue
@@ -54,7 +59,7 @@ def candidates(k, min_tiles_log=8, max_passes=3):
                 avails.append(min(avail, TILE_LOG_MAX - d))
                 log_m -= d
             seen = set()
-            for target in (0, 8, 9, 10, 11, 12):
+            for target in [t for t in (0, 8, 9, 10, 11, 12) if t <= TILE_LOG_MAX]:
                 cols = tuple(min(max(target - d, 0), a) for d, a in zip(digits, avails))
                 if cols in seen:
                     continue
@@ -80,33 +85,57 @@ def time_plan(Z, torch, k, digits, cols, x, y, stream, reps):
     return e0.elapsed_time(e1) / reps
 
 
-def tune(ks, reps=5, out=DEFAULT_OUT, verbose=True, max_candidates=None):
+def time_plan_batch(Z, torch, k, digits, cols, xs, ys, stream, reps):
+    """ms per job of a batch of same-size forward jobs."""
+    Z.plan_set(k, digits, cols, kind=Z.PLAN_BATCH)
+    Z.lib().zkgpu_drop_tables(0)
+    jobs = [Z.make_job(x.data_ptr(), y.data_ptr(), k, Z.OP_FWD) for x, y in zip(xs, ys)]
+    b = Z.Batch(jobs)
+    b.launch(stream.cuda_stream)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        b.launch(stream.cuda_stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    b.close()
+    return e0.elapsed_time(e1) / reps / len(jobs)
+
+
+def tune(ks, reps=5, out=None, verbose=True, max_candidates=None, kind="single"):
     import torch
 
     import zk_gpu_amd as Z
+    out = DEFAULT_OUT[kind] if out is None else out
     dev = torch.device("cuda", 0)
     stream = torch.cuda.current_stream(dev)
     best = {}
     for k in ks:
         n = 1 << k
-        x = torch.empty(n * 4, dtype=torch.int64, device=dev)
-        y = torch.empty_like(x)
-        Z.fill_random(x.data_ptr(), n, 99, stream=stream.cuda_stream)
-        cands = candidates(k)
+        count = 1 if kind == "single" else max(1, (1 << 22) >> k)
+        xs = [torch.empty(n * 4, dtype=torch.int64, device=dev) for _ in range(count)]
+        ys = [torch.empty_like(x) for x in xs]
+        for i, x in enumerate(xs):
+            Z.fill_random(x.data_ptr(), n, 99 + i, stream=stream.cuda_stream)
+        cands = candidates(k, tile_log_max=TILE_LOG[kind])
         if max_candidates:
             cands = cands[:max_candidates]
         results = []
         for digits, cols in cands:
-            ms = time_plan(Z, torch, k, digits, cols, x, y, stream, reps)
+            if kind == "single":
+                ms = time_plan(Z, torch, k, digits, cols, xs[0], ys[0], stream, reps)
+            else:
+                ms = time_plan_batch(Z, torch, k, digits, cols, xs, ys, stream, reps)
             results.append((ms, digits, cols))
             if verbose:
-                print(f"k={k} digits={digits} cols={cols} {ms:.4f} ms", flush=True)
+                print(f"k={k} digits={digits} cols={cols} {ms:.5f} ms", flush=True)
         results.sort()
         best[k] = results[0]
-        Z.plan_reset(k)
+        Z.plan_reset(k, kind=Z.PLAN_SINGLE if kind == "single" else Z.PLAN_BATCH)
         if verbose:
             print(f"BEST k={k}: {results[0]}", flush=True)
-        del x, y
+        del xs, ys
         Z.lib().zkgpu_drop_tables(0)
     if out:
         existing = {}
@@ -120,7 +149,8 @@ def tune(ks, reps=5, out=DEFAULT_OUT, verbose=True, max_candidates=None):
             c = cols + [0] * (3 - len(cols))
             existing[k] = f"{k} {len(digits)} {d[0]} {d[1]} {d[2]} {c[0]} {c[1]} {c[2]} {ms:.5f}"
         with open(out, "w") as f:
-            f.write("# log_n npass d0 d1 d2 c0 c1 c2 ms  (autotuned on gfx950, forward transform)\n")
+            what = "forward transform" if kind == "single" else "batched forward transforms, ms per job"
+            f.write(f"# log_n npass d0 d1 d2 c0 c1 c2 ms  (autotuned on gfx950, {what})\n")
             for k in sorted(existing):
                 f.write(existing[k] + "\n")
     return best
@@ -141,10 +171,11 @@ if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--k", default="12-24")
     ap.add_argument("--reps", type=int, default=5)
-    ap.add_argument("--out", default=DEFAULT_OUT)
+    ap.add_argument("--out", default=None)
+    ap.add_argument("--kind", choices=["single", "batch"], default="single")
     ap.add_argument("--max-candidates", type=int, default=None)
     a = ap.parse_args()
     sys.path.insert(0, os.path.dirname(HERE))
     t = time.time()
-    tune(_parse_ks(a.k), reps=a.reps, out=a.out, max_candidates=a.max_candidates)
+    tune(_parse_ks(a.k), reps=a.reps, out=a.out, max_candidates=a.max_candidates, kind=a.kind)
     print(f"autotune took {time.time() - t:.1f} s")

@@ -207,8 +207,12 @@ int zkgpu_check_canonical(int device_id, const void* d_in, uint64_t count, uint3
 }
 
 int zkgpu_plan_get(uint32_t log_n, uint32_t* digits, uint32_t* log_cols) {
-  if (log_n < 1 || log_n > MAX_LOG_N) return 0;
-  const Plan p = plan_for(log_n);
+  return zkgpu_plan_get_kind(log_n, 0, digits, log_cols);
+}
+
+int zkgpu_plan_get_kind(uint32_t log_n, int kind, uint32_t* digits, uint32_t* log_cols) {
+  if (log_n < 1 || log_n > MAX_LOG_N || kind < 0 || kind > 1) return 0;
+  const Plan p = plan_for(log_n, kind);
   for (uint32_t i = 0; i < MAX_PASSES; ++i) {
     if (digits) digits[i] = p.digit[i];
     if (log_cols) log_cols[i] = p.log_c[i];
@@ -217,11 +221,15 @@ int zkgpu_plan_get(uint32_t log_n, uint32_t* digits, uint32_t* log_cols) {
 }
 
 int zkgpu_plan_set(uint32_t log_n, uint32_t npass, const uint32_t* digits, const uint32_t* log_cols) {
-  if (log_n < 1 || log_n > MAX_LOG_N) return ZK_ERR_INVALID;
+  return zkgpu_plan_set_kind(log_n, 0, npass, digits, log_cols);
+}
+
+int zkgpu_plan_set_kind(uint32_t log_n, int kind, uint32_t npass, const uint32_t* digits, const uint32_t* log_cols) {
+  if (log_n < 1 || log_n > MAX_LOG_N || kind < 0 || kind > 1) return ZK_ERR_INVALID;
   if (npass == 0) {
     Plan d;
-    default_plan(log_n, d);
-    set_plan_override(d);
+    default_plan(log_n, d, plan_tile_log(kind));
+    set_plan_override(d, kind);
     return ZK_OK;
   }
   if (npass > MAX_PASSES || !digits || !log_cols) return ZK_ERR_INVALID;
@@ -232,8 +240,8 @@ int zkgpu_plan_set(uint32_t log_n, uint32_t npass, const uint32_t* digits, const
     p.digit[i] = digits[i];
     p.log_c[i] = log_cols[i];
   }
-  if (!plan_valid(p)) return ZK_ERR_INVALID;
-  set_plan_override(p);
+  if (!plan_valid(p, plan_tile_log(kind))) return ZK_ERR_INVALID;
+  set_plan_override(p, kind);
   return ZK_OK;
 }
 

@@ -13,14 +13,16 @@ namespace zkgpu {
 // ---------------------------------------------------------------- plans
 
 static std::mutex g_plan_mu;
-static std::map<uint32_t, Plan> g_plan_override;
+static std::map<uint32_t, Plan> g_plan_override[2];
 
-bool plan_valid(const Plan& p) {
+uint32_t plan_tile_log(int kind) { return kind == PLAN_BATCH ? batch_tile_log() : TILE_LOG_MAX; }
+
+bool plan_valid(const Plan& p, uint32_t max_tile_log) {
   if (p.log_n < 1 || p.log_n > MAX_LOG_N || p.npass < 1 || p.npass > MAX_PASSES) return false;
   uint32_t sum = 0;
   for (uint32_t i = 0; i < p.npass; ++i) {
-    if (p.digit[i] < 1 || p.digit[i] > TILE_LOG_MAX) return false;
-    if (p.digit[i] + p.log_c[i] > TILE_LOG_MAX) return false;
+    if (p.digit[i] < 1 || p.digit[i] > max_tile_log) return false;
+    if (p.digit[i] + p.log_c[i] > max_tile_log) return false;
     sum += p.digit[i];
   }
   if (sum != p.log_n) return false;
@@ -35,48 +37,51 @@ bool plan_valid(const Plan& p) {
   return p.log_c[p.npass - 1] <= p.digit[0];
 }
 
-bool default_plan(uint32_t log_n, Plan& p) {
+bool default_plan(uint32_t log_n, Plan& p, uint32_t max_tile_log) {
   p = Plan();
   if (log_n < 1 || log_n > MAX_LOG_N) return false;
   p.log_n = log_n;
-  if (log_n <= TILE_LOG_MAX) {
+  if (log_n <= max_tile_log) {
     p.npass = 1;
     p.digit[0] = log_n;
     return true;
   }
-  p.npass = log_n <= 2 * TILE_LOG_MAX ? 2 : 3;
+  p.npass = log_n <= 2 * max_tile_log ? 2 : 3;
   uint32_t rem = log_n;
   for (uint32_t i = 0; i < p.npass; ++i) {
     const uint32_t left = p.npass - i;
     p.digit[i] = (rem + left - 1) / left;
     rem -= p.digit[i];
   }
-  // columns per tile: tiles of at most 4096 elements, at least ~1024 tiles
-  // per pass when the size allows (256 CUs, several tiles each).
+  // columns per tile: tiles of at most 2^max_tile_log elements, at least
+  // ~1024 tiles per pass when the size allows (256 CUs, several tiles each).
   uint32_t log_m = log_n;
   for (uint32_t i = 0; i < p.npass; ++i) {
     const uint32_t avail = (i + 1 < p.npass) ? log_m - p.digit[i] : p.digit[0];
     const uint32_t log_dfts = log_n - p.digit[i];
     uint32_t c = log_dfts > 10 ? log_dfts - 10 : 0;
-    c = std::min(c, std::min(avail, TILE_LOG_MAX - p.digit[i]));
+    c = std::min(c, std::min(avail, max_tile_log - p.digit[i]));
     p.log_c[i] = c;
     log_m -= p.digit[i];
   }
-  return plan_valid(p);
+  return plan_valid(p, max_tile_log);
 }
 
-// Tuned splits: lines "log_n npass d0 d1 d2 c0 c1 c2 [ms]" in the file named by
-// ZKGPU_PLAN_FILE, else plans_gfx950.txt next to this library.
-static void load_plan_file() {
+// Tuned splits: lines "log_n npass d0 d1 d2 c0 c1 c2 [ms]" in the files named
+// by ZKGPU_PLAN_FILE / ZKGPU_BATCH_PLAN_FILE, else plans_gfx950.txt and
+// plans_batch_gfx950.txt next to this library.
+static void load_plan_file(int kind) {
+  const char* env = getenv(kind == PLAN_BATCH ? "ZKGPU_BATCH_PLAN_FILE" : "ZKGPU_PLAN_FILE");
+  const char* name = kind == PLAN_BATCH ? "/plans_batch_gfx950.txt" : "/plans_gfx950.txt";
   std::string path;
-  if (const char* e = getenv("ZKGPU_PLAN_FILE")) {
-    path = e;
+  if (env) {
+    path = env;
   } else {
     Dl_info info;
     if (dladdr((void*)&load_plan_file, &info) && info.dli_fname) {
       path = info.dli_fname;
       const size_t slash = path.rfind('/');
-      path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) + "/plans_gfx950.txt";
+      path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) + name;
     }
   }
   if (path.empty()) return;
@@ -95,34 +100,43 @@ static void load_plan_file() {
       p.digit[i] = v[2 + i];
       p.log_c[i] = v[5 + i];
     }
-    if (plan_valid(p)) g_plan_override[p.log_n] = p;
+    if (plan_valid(p, plan_tile_log(kind))) g_plan_override[kind][p.log_n] = p;
   }
   fclose(f);
 }
 
-Plan plan_for(uint32_t log_n) {
+static void load_plan_files_once() {
   static std::once_flag once;
   std::call_once(once, [] {
     std::lock_guard<std::mutex> g(g_plan_mu);
-    load_plan_file();
+    load_plan_file(PLAN_SINGLE);
+    load_plan_file(PLAN_BATCH);
   });
+}
+
+Plan plan_for(uint32_t log_n, int kind) {
+  kind = kind == PLAN_BATCH ? PLAN_BATCH : PLAN_SINGLE;
+  load_plan_files_once();
   {
     std::lock_guard<std::mutex> g(g_plan_mu);
-    auto it = g_plan_override.find(log_n);
-    if (it != g_plan_override.end()) return it->second;
+    auto it = g_plan_override[kind].find(log_n);
+    if (it != g_plan_override[kind].end()) return it->second;
   }
   Plan p;
-  default_plan(log_n, p);
+  default_plan(log_n, p, plan_tile_log(kind));
   return p;
 }
-void set_plan_override(const Plan& p) {
-  (void)plan_for(p.log_n);  // load the plan file first so it cannot overwrite this override
+void set_plan_override(const Plan& p, int kind) {
+  kind = kind == PLAN_BATCH ? PLAN_BATCH : PLAN_SINGLE;
+  load_plan_files_once();  // the files must not overwrite this override later
   std::lock_guard<std::mutex> g(g_plan_mu);
-  g_plan_override[p.log_n] = p;
+  g_plan_override[kind][p.log_n] = p;
 }
 void clear_plan_overrides() {
+  load_plan_files_once();
   std::lock_guard<std::mutex> g(g_plan_mu);
-  g_plan_override.clear();
+  g_plan_override[0].clear();
+  g_plan_override[1].clear();
 }
 
 // ---------------------------------------------------------------- engine
@@ -301,11 +315,11 @@ int Engine::build_pow_lohi(const host::Fr& base, uint64_t step, uint32_t log_cou
   return ZK_OK;
 }
 
-int Engine::prepare(const JobSpec& job, JobTables& t, hipStream_t s) {
+int Engine::prepare(const JobSpec& job, JobTables& t, hipStream_t s, int plan_kind) {
   std::lock_guard<std::mutex> build_lock(build_mu_);
   t = JobTables();
-  t.plan = plan_for(job.log_n);
-  if (!plan_valid(t.plan)) return ZK_ERR_INVALID;
+  t.plan = plan_for(job.log_n, plan_kind);
+  if (!plan_valid(t.plan, plan_tile_log(plan_kind))) return ZK_ERR_INVALID;
   const Plan& p = t.plan;
   const uint32_t k = job.log_n;
   std::vector<std::shared_ptr<DevBuf>> temps;  // kept alive until the build is synchronised

@@ -49,11 +49,16 @@ struct Plan {
   uint32_t log_c[MAX_PASSES] = {0, 0, 0};
 };
 
-bool default_plan(uint32_t log_n, Plan& out);
-bool plan_valid(const Plan& p);
+// Plans are kept per tile limit: single transforms may use tiles of up to
+// 2^12 elements, batched passes (one persistent kernel for every size) up to
+// 2^11.
+enum PlanKind : int { PLAN_SINGLE = 0, PLAN_BATCH = 1 };
+uint32_t plan_tile_log(int kind);
+bool default_plan(uint32_t log_n, Plan& out, uint32_t max_tile_log = TILE_LOG_MAX);
+bool plan_valid(const Plan& p, uint32_t max_tile_log = TILE_LOG_MAX);
 // The plan used for log_n: an installed override (autotuner) or the default.
-Plan plan_for(uint32_t log_n);
-void set_plan_override(const Plan& p);
+Plan plan_for(uint32_t log_n, int kind = PLAN_SINGLE);
+void set_plan_override(const Plan& p, int kind = PLAN_SINGLE);
 void clear_plan_overrides();
 
 // One transform, device-resident operands.
@@ -112,7 +117,7 @@ class Engine {
 
   // Fetch or build every table a job needs (stream-ordered on `s`, then
   // synchronised once so that the tables are safe for every stream).
-  int prepare(const JobSpec& job, JobTables& t, hipStream_t s);
+  int prepare(const JobSpec& job, JobTables& t, hipStream_t s, int plan_kind = PLAN_SINGLE);
   // Pass descriptors of a prepared job; ws must hold 36 * 2^log_n bytes (planar
   // limb form) when the plan has more than one pass.
   int build_descs(const JobSpec& job, const JobTables& t, void* ws, PassDesc* descs, uint32_t* ndesc) const;
@@ -149,9 +154,6 @@ class Engine {
 // HIP launchers (ntt_kernels.hip)
 hipError_t configure_kernels();
 hipError_t launch_pass(const PassDesc& d, hipStream_t s);
-hipError_t launch_pass_batched(uint32_t log_n_tile, const PassDesc* d_descs, const uint32_t* d_tile_start,
-                               uint32_t ndesc, uint32_t total_tiles, uint32_t block_threads, uint32_t lds_bytes,
-                               hipStream_t s);
 hipError_t launch_pow_table(void* out, const uint32_t base[8], uint64_t step, uint32_t count, bool limbs48,
                             hipStream_t s);
 hipError_t launch_pow_table_2level(void* out, const void* lo, const void* hi, uint32_t h, uint64_t count,
@@ -160,6 +162,10 @@ hipError_t launch_twiddle_table(void* out, const void* lo, const void* hi, uint3
                                 uint32_t log_L, hipStream_t s);
 hipError_t launch_fill_random(void* out, uint64_t seed, uint64_t start, uint64_t count, hipStream_t s);
 hipError_t launch_check_canonical(const void* in, uint64_t count, uint32_t* flag, hipStream_t s);
+hipError_t launch_persistent(const PassDesc* d_descs, const uint32_t* d_tile_start, uint32_t ndesc, uint32_t total,
+                             uint32_t* d_counter, uint32_t grid, hipStream_t s);
+hipError_t launch_reset_counters(uint32_t* d_counter, uint32_t n, hipStream_t s);
+uint32_t batch_tile_log();
 uint32_t tile_threads(uint32_t log_n_tile, uint32_t log_c);
 uint32_t tile_lds_bytes(uint32_t log_n_tile, uint32_t log_c);
 

@@ -410,22 +410,50 @@ __global__ void __launch_bounds__(MAX_TILE_THREADS) ntt_pass_kernel(PassDesc d) 
   ntt_tile<LOGN>(d, xcd_tile(blockIdx.x, d.num_tiles), lds);
 }
 
-// Many jobs' passes in one launch, all with tile length 2^LOGN: tile_start is
-// the exclusive prefix sum of the descriptors' tile counts; blockDim is the
-// largest tile's thread count (smaller tiles idle their extra threads).
-template <int LOGN>
-__global__ void __launch_bounds__(MAX_TILE_THREADS) ntt_pass_batched_kernel(const PassDesc* __restrict__ descs,
-                                                                           const uint32_t* __restrict__ tile_start,
-                                                                           uint32_t ndesc, uint32_t total_tiles) {
-  extern __shared__ uint32_t lds[];
-  const uint32_t b = xcd_tile(blockIdx.x, total_tiles);
-  uint32_t lo = 0, hi = ndesc;  // largest idx with tile_start[idx] <= b
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (tile_start[mid] <= b) lo = mid; else hi = mid;
+// Persistent pass kernel for batches: a grid of resident workgroups drains a
+// queue of tiles (any job, any tile length up to 2^BATCH_TILE_LOG elements)
+// through an atomic counter. Items are ordered by descriptor; tile_start is
+// the exclusive prefix sum of the descriptors' tile counts. Every workgroup
+// leaves the loop once the counter passes `total`.
+constexpr uint32_t BATCH_TILE_LOG = 11;  // 2048-element tiles: 72 KiB of LDS, two workgroups per CU
+constexpr int BATCH_THREADS = 1 << (BATCH_TILE_LOG - 3);
+
+template <int L>
+__device__ __forceinline__ void tile_dispatch(const PassDesc& d, uint32_t tile, uint32_t* lds) {
+  if constexpr (L <= (int)BATCH_TILE_LOG) {
+    if (d.log_n_tile == (uint32_t)L) {
+      ntt_tile<L>(d, tile, lds);
+      return;
+    }
+    tile_dispatch<L + 1>(d, tile, lds);
   }
-  const PassDesc d = descs[lo];
-  ntt_tile<LOGN>(d, b - tile_start[lo], lds);
+}
+
+__global__ void __launch_bounds__(BATCH_THREADS, 2) ntt_persistent_kernel(const PassDesc* __restrict__ descs,
+                                                                       const uint32_t* __restrict__ tile_start,
+                                                                       uint32_t ndesc, uint32_t total,
+                                                                       uint32_t* counter) {
+  extern __shared__ uint32_t lds[];
+  __shared__ uint32_t item_s;
+  for (;;) {
+    if (threadIdx.x == 0) item_s = atomicAdd(counter, 1u);
+    __syncthreads();
+    const uint32_t item = item_s;
+    __syncthreads();  // every thread has read item_s before it is overwritten
+    if (item >= total) break;
+    uint32_t lo = 0, hi = ndesc;  // largest idx with tile_start[idx] <= item
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (tile_start[mid] <= item) lo = mid; else hi = mid;
+    }
+    const PassDesc d = descs[lo];
+    tile_dispatch<1>(d, item - tile_start[lo], lds);
+  }
+}
+
+__global__ void reset_counters_kernel(uint32_t* c, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) c[i] = 0;
 }
 
 // ---------------------------------------------------------------- tables
@@ -562,12 +590,6 @@ template <int LOGN>
 static void launch_one(const PassDesc& d, hipStream_t s) {
   ntt_pass_kernel<LOGN><<<d.num_tiles, tile_threads(LOGN, d.log_c), tile_lds_bytes(LOGN, d.log_c), s>>>(d);
 }
-template <int LOGN>
-static void launch_batched_one(const PassDesc* descs, const uint32_t* ts, uint32_t ndesc, uint32_t total,
-                               uint32_t threads, uint32_t lds, hipStream_t s) {
-  ntt_pass_batched_kernel<LOGN><<<total, threads, lds, s>>>(descs, ts, ndesc, total);
-}
-
 #define ZK_TILE_CASES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12)
 
 hipError_t launch_pass(const PassDesc& d, hipStream_t s) {
@@ -580,18 +602,17 @@ hipError_t launch_pass(const PassDesc& d, hipStream_t s) {
   return hipGetLastError();
 }
 
-// All descriptors of one batched launch share log_n_tile.
-hipError_t launch_pass_batched(uint32_t log_n_tile, const PassDesc* d_descs, const uint32_t* d_tile_start,
-                               uint32_t ndesc, uint32_t total_tiles, uint32_t block_threads, uint32_t lds_bytes,
-                               hipStream_t s) {
-  switch (log_n_tile) {
-#define X(L) case L: launch_batched_one<L>(d_descs, d_tile_start, ndesc, total_tiles, block_threads, lds_bytes, s); break;
-    ZK_TILE_CASES(X)
-#undef X
-    default: return hipErrorInvalidValue;
-  }
+hipError_t launch_persistent(const PassDesc* d_descs, const uint32_t* d_tile_start, uint32_t ndesc, uint32_t total,
+                             uint32_t* d_counter, uint32_t grid, hipStream_t s) {
+  ntt_persistent_kernel<<<grid, BATCH_THREADS, tile_lds_bytes(BATCH_TILE_LOG, 0), s>>>(d_descs, d_tile_start, ndesc,
+                                                                                     total, d_counter);
   return hipGetLastError();
 }
+hipError_t launch_reset_counters(uint32_t* d_counter, uint32_t n, hipStream_t s) {
+  reset_counters_kernel<<<(n + 63) / 64, 64, 0, s>>>(d_counter, n);
+  return hipGetLastError();
+}
+uint32_t batch_tile_log() { return BATCH_TILE_LOG; }
 
 static FeArg to_arg(const uint32_t v[8]) {
   FeArg a;
@@ -633,11 +654,11 @@ hipError_t configure_kernels() {
   if (e == hipSuccess)                                                                                    \
     e = hipFuncSetAttribute((const void*)ntt_pass_kernel<L>, hipFuncAttributeMaxDynamicSharedMemorySize, \
                             max_lds);                                                                     \
-  if (e == hipSuccess)                                                                                    \
-    e = hipFuncSetAttribute((const void*)ntt_pass_batched_kernel<L>,                                      \
-                            hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
   ZK_TILE_CASES(X)
 #undef X
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void*)ntt_persistent_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)tile_lds_bytes(BATCH_TILE_LOG, 0));
   return e;
 }
 
