@@ -70,6 +70,59 @@ def cpu_baseline(budget_s=15.0):
                       f"log_split=floor(log2 {cores}))"}
 
 
+def bench_batched(torch, Z, dev, stream, count=64, seed=4, reps=10):
+    """64 polynomials, sizes log-uniform in 2^12..2^20, ops uniform over
+    {fwd, inv, coset_fwd, coset_inv}: the captured batch vs the plain path
+    (one device-resident call per job, synchronised after each call, and the
+    same calls issued back to back)."""
+    import random
+    rnd = random.Random(seed)
+    sptr = stream.cuda_stream
+    jobs_spec = [(rnd.randint(12, 20), rnd.choice([Z.OP_FWD, Z.OP_INV, Z.OP_COSET_FWD, Z.OP_COSET_INV]))
+                 for _ in range(count)]
+    xs, ys, jobs = [], [], []
+    for i, (k, op) in enumerate(jobs_spec):
+        x = torch.empty((1 << k) * 4, dtype=torch.int64, device=dev)
+        Z.fill_random(x.data_ptr(), 1 << k, 40_000 + i, stream=sptr)
+        y = torch.empty_like(x)
+        xs.append(x)
+        ys.append(y)
+        jobs.append(Z.make_job(x.data_ptr(), y.data_ptr(), k, op))
+        Z.prepare_std(k, op)
+    elements = sum(1 << k for k, _ in jobs_spec)
+    b = Z.Batch(jobs)
+    b.launch(sptr)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        b.launch(sptr)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    batch_ms = e0.elapsed_time(e1) / reps
+
+    def plain(sync_each):
+        t0 = time.perf_counter()
+        for (k, op), x, y in zip(jobs_spec, xs, ys):
+            Z.ntt_std_device(x.data_ptr(), y.data_ptr(), k, op, stream=sptr)
+            if sync_each:
+                torch.cuda.synchronize()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3
+
+    plain(True)
+    plain_sync_ms = min(plain(True) for _ in range(3))
+    plain_async_ms = min(plain(False) for _ in range(3))
+    info = b.info()
+    b.close()
+    return {"config": "64 polynomials, log2 sizes uniform in 12..20, ops uniform over fwd/inv/coset_fwd/coset_inv",
+            "elements": elements, "levels": info["levels"], "tiles": info["tiles"],
+            "batch_ms": batch_ms, "batch_elem_per_s": elements / (batch_ms * 1e-3),
+            "plain_sync_ms": plain_sync_ms, "plain_sync_elem_per_s": elements / (plain_sync_ms * 1e-3),
+            "plain_async_ms": plain_async_ms, "plain_async_elem_per_s": elements / (plain_async_ms * 1e-3),
+            "speedup_vs_plain_sync": plain_sync_ms / batch_ms, "speedup_vs_plain_async": plain_async_ms / batch_ms}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +130,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-k24", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (--gpus {args.gpus} requested); "
@@ -182,6 +236,9 @@ def main():
                                        "peak": PEAK_HBM_GBS, "unit": "GB/s",
                                        "frac": 64 * n24 / (l_ms * 1e-3) / 1e9 / PEAK_HBM_GBS}}
         del x24, y24
+
+    if not args.no_batch:
+        res["batched"] = bench_batched(torch, Z, dev, stream)
 
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline()

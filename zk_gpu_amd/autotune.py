@@ -9,11 +9,13 @@ Results are written as plain text lines
 
 to zk_gpu_amd/plans_gfx950.txt (single transforms, tiles up to 2^12
 elements) or zk_gpu_amd/plans_batch_gfx950.txt (batched transforms, tiles up
-to 2^11 elements, timed as a batch of same-size jobs of about 2^22 elements
-in total), which the library reads at start-up (ZKGPU_PLAN_FILE /
+to 2^11 elements). Batched plans are tuned on a mixed workload (--mix): a
+batch of jobs of several sizes and ops is timed as a whole while the split of
+one size class at a time is varied (coordinate descent over the classes), which the library reads at start-up (ZKGPU_PLAN_FILE /
 ZKGPU_BATCH_PLAN_FILE override the paths).
 
-    python -m zk_gpu_amd.autotune --k 12-24 [--kind single|batch] [--reps 5] [--out PATH]
+    python -m zk_gpu_amd.autotune --k 12-24 [--reps 5] [--out PATH]
+    python -m zk_gpu_amd.autotune --kind batch --mix mixed64 [--sweeps 2]
 """
 import argparse
 import itertools
@@ -156,6 +158,88 @@ def tune(ks, reps=5, out=None, verbose=True, max_candidates=None, kind="single")
     return best
 
 
+MIXES = {
+    # 64 polynomials, log2 sizes uniform in 12..20, ops uniform over the four ops
+    "mixed64": dict(count=64, lo=12, hi=20, seed=4),
+    # serving-like jobs, log2 sizes uniform in 14..22
+    "serve": dict(count=48, lo=14, hi=22, seed=5),
+}
+
+
+def tune_batch_mix(mix="mixed64", reps=3, sweeps=2, out=None, verbose=True):
+    """Coordinate descent over the size classes of a mixed batch: for each
+    class in turn, try every candidate split with the other classes fixed and
+    keep the one that makes the whole batch fastest."""
+    import random
+
+    import torch
+
+    import zk_gpu_amd as Z
+    out = DEFAULT_OUT["batch"] if out is None else out
+    cfg = MIXES[mix]
+    rnd = random.Random(cfg["seed"])
+    spec = [(rnd.randint(cfg["lo"], cfg["hi"]), rnd.choice([0, 1, 2, 3])) for _ in range(cfg["count"])]
+    dev = torch.device("cuda", 0)
+    stream = torch.cuda.current_stream(dev)
+    xs, ys = [], []
+    for i, (k, _) in enumerate(spec):
+        x = torch.empty((1 << k) * 4, dtype=torch.int64, device=dev)
+        Z.fill_random(x.data_ptr(), 1 << k, 70_000 + i, stream=stream.cuda_stream)
+        xs.append(x)
+        ys.append(torch.empty_like(x))
+    jobs = [Z.make_job(x.data_ptr(), y.data_ptr(), k, op) for (k, op), x, y in zip(spec, xs, ys)]
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def run():
+        b = Z.Batch(jobs)
+        b.launch(stream.cuda_stream)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(reps):
+            b.launch(stream.cuda_stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        b.close()
+        return e0.elapsed_time(e1) / reps
+
+    classes = sorted(set(k for k, _ in spec))
+    current = {k: Z.plan_get(k, Z.PLAN_BATCH)[1:] for k in classes}
+    best_ms = run()
+    if verbose:
+        print(f"start {best_ms:.4f} ms", flush=True)
+    for sweep in range(sweeps):
+        for k in classes:
+            for digits, cols in candidates(k, tile_log_max=TILE_LOG["batch"]):
+                Z.plan_set(k, digits, cols, kind=Z.PLAN_BATCH)
+                ms = run()
+                if ms < best_ms:
+                    best_ms = ms
+                    current[k] = (digits, cols)
+                    if verbose:
+                        print(f"sweep {sweep} k={k} -> {digits} {cols}: {ms:.4f} ms", flush=True)
+            Z.plan_set(k, current[k][0], current[k][1], kind=Z.PLAN_BATCH)
+            Z.lib().zkgpu_drop_tables(0)
+    if verbose:
+        print(f"BEST mix={mix}: {best_ms:.4f} ms", flush=True)
+    if out:
+        existing = {}
+        if os.path.exists(out):
+            for line in open(out):
+                f = line.split()
+                if len(f) >= 9 and not line.startswith("#"):
+                    existing[int(f[0])] = line.rstrip("\n")
+        for k, (digits, cols) in current.items():
+            d = list(digits) + [0] * (3 - len(digits))
+            c = list(cols) + [0] * (3 - len(cols))
+            existing[k] = f"{k} {len(digits)} {d[0]} {d[1]} {d[2]} {c[0]} {c[1]} {c[2]} {best_ms:.5f}"
+        with open(out, "w") as f:
+            f.write(f"# log_n npass d0 d1 d2 c0 c1 c2 ms  (autotuned on gfx950 for batches, mix {mix}: "
+                    f"whole-batch ms)\n")
+            for k in sorted(existing):
+                f.write(existing[k] + "\n")
+    return current, best_ms
+
+
 def _parse_ks(s):
     ks = []
     for part in s.split(","):
@@ -174,8 +258,13 @@ if __name__ == "__main__":
     ap.add_argument("--out", default=None)
     ap.add_argument("--kind", choices=["single", "batch"], default="single")
     ap.add_argument("--max-candidates", type=int, default=None)
+    ap.add_argument("--mix", default=None, help="tune batch plans on a mixed workload (mixed64, serve)")
+    ap.add_argument("--sweeps", type=int, default=2)
     a = ap.parse_args()
     sys.path.insert(0, os.path.dirname(HERE))
     t = time.time()
-    tune(_parse_ks(a.k), reps=a.reps, out=a.out, max_candidates=a.max_candidates, kind=a.kind)
+    if a.mix:
+        tune_batch_mix(a.mix, reps=a.reps, sweeps=a.sweeps, out=a.out)
+    else:
+        tune(_parse_ks(a.k), reps=a.reps, out=a.out, max_candidates=a.max_candidates, kind=a.kind)
     print(f"autotune took {time.time() - t:.1f} s")

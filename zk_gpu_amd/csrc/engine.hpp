@@ -49,9 +49,9 @@ struct Plan {
   uint32_t log_c[MAX_PASSES] = {0, 0, 0};
 };
 
-// Plans are kept per tile limit: single transforms may use tiles of up to
-// 2^12 elements, batched passes (one persistent kernel for every size) up to
-// 2^11.
+// Plans are kept per use: single transforms may use tiles of up to 2^12
+// elements, batched passes (one persistent kernel for every size) up to 2^11;
+// the two tables are tuned separately.
 enum PlanKind : int { PLAN_SINGLE = 0, PLAN_BATCH = 1 };
 uint32_t plan_tile_log(int kind);
 bool default_plan(uint32_t log_n, Plan& out, uint32_t max_tile_log = TILE_LOG_MAX);

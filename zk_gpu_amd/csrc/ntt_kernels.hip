@@ -415,8 +415,13 @@ __global__ void __launch_bounds__(MAX_TILE_THREADS) ntt_pass_kernel(PassDesc d) 
 // through an atomic counter. Items are ordered by descriptor; tile_start is
 // the exclusive prefix sum of the descriptors' tile counts. Every workgroup
 // leaves the loop once the counter passes `total`.
+//
+// LDS layout: the tile's three limb planes from byte 0 (16-byte aligned, as
+// in the single-pass kernel: the planes are accessed with 16-byte DS
+// operations), then one word holding the claimed item.
 constexpr uint32_t BATCH_TILE_LOG = 11;  // 2048-element tiles: 72 KiB of LDS, two workgroups per CU
 constexpr int BATCH_THREADS = 1 << (BATCH_TILE_LOG - 3);
+constexpr uint32_t BATCH_ITEM_WORD = NLIMBS << BATCH_TILE_LOG;  // LDS word index of the item slot
 
 template <int L>
 __device__ __forceinline__ void tile_dispatch(const PassDesc& d, uint32_t tile, uint32_t* lds) {
@@ -433,13 +438,14 @@ __global__ void __launch_bounds__(BATCH_THREADS, 2) ntt_persistent_kernel(const 
                                                                        const uint32_t* __restrict__ tile_start,
                                                                        uint32_t ndesc, uint32_t total,
                                                                        uint32_t* counter) {
-  extern __shared__ uint32_t lds[];
-  __shared__ uint32_t item_s;
+  extern __shared__ __align__(16) uint32_t lds[];
   for (;;) {
-    if (threadIdx.x == 0) item_s = atomicAdd(counter, 1u);
+    // the slot lies past every tile's planes, and all reads of its previous
+    // value precede the second barrier of the previous iteration
+    if (threadIdx.x == 0) lds[BATCH_ITEM_WORD] = atomicAdd(counter, 1u);
     __syncthreads();
-    const uint32_t item = item_s;
-    __syncthreads();  // every thread has read item_s before it is overwritten
+    const uint32_t item = lds[BATCH_ITEM_WORD];
+    __syncthreads();  // every thread has read the slot before the tile reuses LDS
     if (item >= total) break;
     uint32_t lo = 0, hi = ndesc;  // largest idx with tile_start[idx] <= item
     while (hi - lo > 1) {
@@ -585,6 +591,8 @@ uint32_t tile_threads(uint32_t log_n_tile, uint32_t log_c) {
 uint32_t tile_lds_bytes(uint32_t log_n_tile, uint32_t log_c) {
   return (uint32_t)(4u * NLIMBS) << (log_n_tile + log_c);
 }
+// planes of the largest batch tile, then the 16-byte item slot
+static uint32_t persistent_lds_bytes() { return 4u * BATCH_ITEM_WORD + 16u; }
 
 template <int LOGN>
 static void launch_one(const PassDesc& d, hipStream_t s) {
@@ -604,7 +612,7 @@ hipError_t launch_pass(const PassDesc& d, hipStream_t s) {
 
 hipError_t launch_persistent(const PassDesc* d_descs, const uint32_t* d_tile_start, uint32_t ndesc, uint32_t total,
                              uint32_t* d_counter, uint32_t grid, hipStream_t s) {
-  ntt_persistent_kernel<<<grid, BATCH_THREADS, tile_lds_bytes(BATCH_TILE_LOG, 0), s>>>(d_descs, d_tile_start, ndesc,
+  ntt_persistent_kernel<<<grid, BATCH_THREADS, persistent_lds_bytes(), s>>>(d_descs, d_tile_start, ndesc,
                                                                                      total, d_counter);
   return hipGetLastError();
 }
@@ -658,7 +666,7 @@ hipError_t configure_kernels() {
 #undef X
   if (e == hipSuccess)
     e = hipFuncSetAttribute((const void*)ntt_persistent_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)tile_lds_bytes(BATCH_TILE_LOG, 0));
+                            (int)persistent_lds_bytes());
   return e;
 }
 
