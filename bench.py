@@ -123,6 +123,155 @@ def bench_batched(torch, Z, dev, stream, count=64, seed=4, reps=10):
             "speedup_vs_plain_sync": plain_sync_ms / batch_ms, "speedup_vs_plain_async": plain_async_ms / batch_ms}
 
 
+def _serve_spec(count, seed=5, lo=14, hi=22):
+    import random
+    rnd = random.Random(seed)
+    return [(rnd.randint(lo, hi), rnd.choice([0, 1, 2, 3])) for _ in range(count)]
+
+
+def _arrivals(count, rate, seed):
+    """Poisson process: exponential inter-arrival times (seconds from start)."""
+    import random
+    rnd = random.Random(seed)
+    t, out = 0.0, []
+    for _ in range(count):
+        t += rnd.expovariate(rate)
+        out.append(t)
+    return out
+
+
+def _wait_until(t_ns):
+    while True:
+        d = t_ns - time.monotonic_ns()
+        if d <= 0:
+            return
+        time.sleep(d / 2e9 if d > 400_000 else 0)
+
+
+def _pct(xs, q):
+    xs = sorted(xs)
+    return xs[min(len(xs) - 1, int(q * len(xs)))]
+
+
+def bench_serving(torch, Z, dev, stream, count=10_000, loads=(0.5, 0.9, 1.5), seed=5):
+    """Poisson stream of `count` jobs, k uniform in 14..22, ops uniform over the
+    four ops, device-resident operands (every job reads a prefix of one random
+    2^22-element input; results go to server-owned buffers, released on
+    completion), HBM admission budget 64 GiB. The plain path serves the same
+    arrivals one call at a time (ntt_std_device + synchronize, FIFO)."""
+    import threading
+    sptr = stream.cuda_stream
+    spec = _serve_spec(count)
+    elements = sum(1 << k for k, _ in spec)
+    src = torch.empty((1 << 22) * 4, dtype=torch.int64, device=dev)
+    Z.fill_random(src.data_ptr(), 1 << 22, 8080, stream=sptr)
+    outs = {k: torch.empty((1 << k) * 4, dtype=torch.int64, device=dev) for k in range(14, 23)}
+    for k in range(14, 23):
+        for op in range(4):
+            Z.prepare_std(k, op)
+    torch.cuda.synchronize()
+
+    def plain_run(arr_ns):
+        lat = []
+        for (k, op), a in zip(spec, arr_ns):
+            if a is not None:
+                _wait_until(a)
+            Z.ntt_std_device(src.data_ptr(), outs[k].data_ptr(), k, op, stream=sptr)
+            stream.synchronize()
+            if a is not None:
+                lat.append(time.monotonic_ns() - a)
+        return lat
+
+    srv = Z.Server(budget_bytes=64 << 30)
+
+    def served_run(arr_ns, check_every=0):
+        tickets = [None] * len(spec)
+        checked = []
+        lat = []
+        t_first = [None]
+
+        def submitter():
+            for i, ((k, op), a) in enumerate(zip(spec, arr_ns)):
+                if a is not None:
+                    _wait_until(a)
+                tickets[i] = srv.submit(Z.make_job(src.data_ptr(), None, k, op))
+
+        th = threading.Thread(target=submitter)
+        t0 = time.monotonic_ns()
+        th.start()
+        last_done = 0
+        for i, ((k, op), a) in enumerate(zip(spec, arr_ns)):
+            while tickets[i] is None:
+                time.sleep(0.0001)
+            r = srv.wait(tickets[i])
+            if t_first[0] is None:
+                t_first[0] = r.t_submit_ns
+            last_done = max(last_done, r.t_done_ns)
+            lat.append(r.t_done_ns - (a if a is not None else r.t_submit_ns))
+            if check_every and i % check_every == 0:
+                checked.append((i, k, op, _device_copy(torch, r.d_out, k)))
+            srv.release(tickets[i])
+        th.join()
+        return lat, (last_done - t0) * 1e-9, checked
+
+    # warm-up: every size class through both paths (pool buffers, tables)
+    warm = [None] * len(spec)
+    plain_run(warm[:200])
+    served_run(warm[:200])
+    # plain capacity: all jobs back to back, one call + synchronize each
+    t0 = time.perf_counter()
+    plain_run([None] * len(spec))
+    plain_s = time.perf_counter() - t0
+    plain_rate = count / plain_s
+    # served capacity: all jobs submitted at once
+    lat_sat, served_s, checked = served_run([None] * len(spec), check_every=max(1, count // 40))
+    # parity: sampled served results equal the plain path's
+    mismatches = 0
+    for i, k, op, got in checked:
+        Z.ntt_std_device(src.data_ptr(), outs[k].data_ptr(), k, op, stream=sptr)
+        stream.synchronize()
+        mismatches += int(not torch.equal(got, outs[k]))
+    res = {"config": f"{count} jobs, k uniform in 14..22, ops uniform over fwd/inv/coset_fwd/coset_inv, "
+                     "Poisson arrivals, HBM admission budget 64 GiB, device-resident operands",
+           "elements": elements,
+           "plain_elem_per_s": elements / plain_s, "plain_jobs_per_s": plain_rate,
+           "served_elem_per_s": elements / served_s, "served_jobs_per_s": count / served_s,
+           "speedup_vs_plain": plain_s / served_s,
+           "saturated_p50_ms": _pct(lat_sat, 0.5) / 1e6, "saturated_p99_ms": _pct(lat_sat, 0.99) / 1e6,
+           "sampled_results_checked": len(checked), "sampled_mismatches": mismatches,
+           "loads": []}
+    for i, rho in enumerate(loads):
+        lam = rho * plain_rate
+        arr = _arrivals(count, lam, seed=100 + i)
+        base = time.monotonic_ns() + 2_000_000
+        arr_ns = [base + int(a * 1e9) for a in arr]
+        lat_p = plain_run(arr_ns)
+        base = time.monotonic_ns() + 2_000_000
+        arr_ns = [base + int(a * 1e9) for a in arr]
+        lat_s, s_s, _ = served_run(arr_ns)
+        res["loads"].append({"offered_jobs_per_s": lam, "load_vs_plain_capacity": rho,
+                             "plain_p50_ms": _pct(lat_p, 0.5) / 1e6, "plain_p99_ms": _pct(lat_p, 0.99) / 1e6,
+                             "served_p50_ms": _pct(lat_s, 0.5) / 1e6, "served_p99_ms": _pct(lat_s, 0.99) / 1e6,
+                             "served_elem_per_s": elements / s_s})
+    st = srv.stats()
+    srv.close()
+    res["server"] = {"iterations": st["iterations"], "peak_in_use_bytes": st["peak_in_use_bytes"],
+                     "pool_bytes": st["pool_bytes"], "failed": st["failed"]}
+    return res
+
+
+def _device_copy(torch, d_ptr, k):
+    """Tensor copy of a server-owned result buffer."""
+    import ctypes
+    n = 1 << k
+    t = torch.empty(n * 4, dtype=torch.int64, device="cuda")
+    hip = ctypes.CDLL("libamdhip64.so")
+    rc = hip.hipMemcpy(ctypes.c_void_p(t.data_ptr()), ctypes.c_void_p(d_ptr), ctypes.c_size_t(n * 32), 3)
+    if rc != 0:
+        raise RuntimeError(f"hipMemcpy failed: {rc}")
+    return t
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -131,6 +280,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-k24", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-jobs", type=int, default=10_000)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="copy 256 MiB once on the device first: a kernel of known traffic for "
+                         "calibrating rocprofv3 FETCH_SIZE/WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (--gpus {args.gpus} requested); "
@@ -143,6 +297,12 @@ def main():
     dev = torch.device("cuda", 0)
     stream = torch.cuda.current_stream(dev)
     sptr = stream.cuda_stream
+    if args.copy_probe:
+        src = torch.ones(1 << 25, dtype=torch.int64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)  # 256 MiB read + 256 MiB written
+        torch.cuda.synchronize()
+        del src, dst
     k = 20
     n = 1 << k
     x = torch.empty(n * 4, dtype=torch.int64, device=dev)
@@ -239,6 +399,9 @@ def main():
 
     if not args.no_batch:
         res["batched"] = bench_batched(torch, Z, dev, stream)
+
+    if not args.no_serve:
+        res["served"] = bench_serving(torch, Z, dev, stream, count=args.serve_jobs)
 
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline()

@@ -160,6 +160,60 @@ int zkgpu_batch_destroy(uint64_t handle);
 /* create + launch + synchronize + destroy. */
 int zkgpu_ntt_batch_device(int device_id, const zkgpu_job* jobs, uint32_t count, void* stream);
 
+/* ---------------------------------------------------------------------
+ * Serving (replaces the reference's per-call admission, GpuLock in
+ * src/gpulock.rs:133-226, for one device): a request loop that takes NTT
+ * jobs as they arrive and runs them with continuous batching. One
+ * scheduler thread per server issues iterations on its own stream; each
+ * iteration is one persistent-kernel launch that runs the next pass of
+ * every admitted job, so a job submitted while others are on the device
+ * joins at the next iteration and leaves after its last pass. Jobs are
+ * admitted in arrival order while their device footprint (output buffer
+ * when the server provides it, plus the 36-byte-per-element workspace of a
+ * multi-pass plan) fits the HBM budget and fewer than max_active jobs are
+ * admitted. Twiddle tables stay resident across jobs. Each result is
+ * bit-identical to the same job run alone.
+ * --------------------------------------------------------------------- */
+
+typedef struct zkgpu_server_result {
+  int32_t status;       /* ZKGPU_OK when done, ZKGPU_ERR_* on failure */
+  int32_t done;         /* 1 once the result is in d_out */
+  void* d_out;          /* the job's d_out, or the server-owned result buffer */
+  uint64_t t_submit_ns; /* steady-clock times: submission, admission, first */
+  uint64_t t_admit_ns;  /* launch of one of its passes, completion seen by */
+  uint64_t t_start_ns;  /* the scheduler */
+  uint64_t t_done_ns;
+} zkgpu_server_result;
+
+typedef struct zkgpu_server_stats {
+  uint64_t submitted, completed, failed;
+  uint64_t pending, active;     /* waiting for admission / admitted, not done */
+  uint64_t iterations;          /* persistent-kernel launches so far */
+  uint64_t budget_bytes, in_use_bytes, peak_in_use_bytes, pool_bytes;
+  uint64_t elements_done;
+} zkgpu_server_stats;
+
+/* Start a server on a device. budget_bytes = 0 selects 64 GiB; max_active =
+ * 0 selects 256 admitted jobs. flags is reserved (0). */
+int zkgpu_server_create(int device_id, uint64_t budget_bytes, uint32_t max_active, uint32_t flags,
+                        uint64_t* handle);
+/* Queue a job (non-blocking). job->d_in must hold its data when the job is
+ * submitted (the work producing it complete: the server runs on its own
+ * stream) and stay valid and unmodified until the job is done; job->d_out = NULL asks for a server-owned result buffer
+ * (valid until zkgpu_server_release). A job whose footprint alone exceeds the
+ * budget returns ZKGPU_ERR_BUDGET. */
+int zkgpu_server_submit(uint64_t handle, const zkgpu_job* job, uint64_t* ticket);
+/* State of a job; wait_ms < 0 waits for completion, 0 polls, > 0 waits at
+ * most that long. */
+int zkgpu_server_poll(uint64_t handle, uint64_t ticket, int32_t wait_ms, zkgpu_server_result* out);
+/* Forget a finished job and return its server-owned result buffer to the
+ * pool. */
+int zkgpu_server_release(uint64_t handle, uint64_t ticket);
+int zkgpu_server_stats_get(uint64_t handle, zkgpu_server_stats* out);
+/* Finish the admitted jobs, fail the pending ones with ZKGPU_ERR_SHUTDOWN,
+ * stop the scheduler and free the pool. */
+int zkgpu_server_destroy(uint64_t handle);
+
 /* Resident table bytes on a device; drop them with zkgpu_drop_tables. */
 uint64_t zkgpu_table_bytes(int device_id);
 int zkgpu_drop_tables(int device_id);

@@ -62,6 +62,12 @@ SIGNATURES = {
     "zkgpu_batch_info": (_I, [_U64, _P, _P, _P]),
     "zkgpu_batch_destroy": (_I, [_U64]),
     "zkgpu_ntt_batch_device": (_I, [_I, _P, _U32, _P]),
+    "zkgpu_server_create": (_I, [_I, _U64, _U32, _U32, _P]),
+    "zkgpu_server_submit": (_I, [_U64, _P, _P]),
+    "zkgpu_server_poll": (_I, [_U64, _U64, ctypes.c_int32, _P]),
+    "zkgpu_server_release": (_I, [_U64, _U64]),
+    "zkgpu_server_stats_get": (_I, [_U64, _P]),
+    "zkgpu_server_destroy": (_I, [_U64]),
     "zkgpu_table_bytes": (_U64, [_I]),
     "zkgpu_drop_tables": (_I, [_I]),
     "zkgpu_error_string": (ctypes.c_char_p, [_I]),
@@ -254,6 +260,67 @@ class Batch:
     def close(self):
         if self.handle:
             lib().zkgpu_batch_destroy(self.handle)
+            self.handle = 0
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------- serving
+
+class ServerResult(ctypes.Structure):
+    """zkgpu_server_result (include/zkgpu_ntt.h)."""
+    _fields_ = [("status", ctypes.c_int32), ("done", ctypes.c_int32), ("d_out", ctypes.c_void_p),
+                ("t_submit_ns", ctypes.c_uint64), ("t_admit_ns", ctypes.c_uint64),
+                ("t_start_ns", ctypes.c_uint64), ("t_done_ns", ctypes.c_uint64)]
+
+
+class ServerStats(ctypes.Structure):
+    """zkgpu_server_stats (include/zkgpu_ntt.h)."""
+    _fields_ = [(name, ctypes.c_uint64) for name in (
+        "submitted", "completed", "failed", "pending", "active", "iterations", "budget_bytes",
+        "in_use_bytes", "peak_in_use_bytes", "pool_bytes", "elements_done")]
+
+
+class Server:
+    """Continuous-batching NTT server on one device (jobs as make_job();
+    d_out = None asks for a server-owned result buffer)."""
+
+    def __init__(self, budget_bytes=0, max_active=0, device_id=0):
+        h = ctypes.c_uint64(0)
+        check(lib().zkgpu_server_create(device_id, budget_bytes, max_active, 0, ctypes.byref(h)),
+              "zkgpu_server_create")
+        self.handle = h.value
+
+    def submit(self, job):
+        t = ctypes.c_uint64(0)
+        check(lib().zkgpu_server_submit(self.handle, ctypes.byref(job), ctypes.byref(t)), "zkgpu_server_submit")
+        return t.value
+
+    def poll(self, ticket, wait_ms=0):
+        r = ServerResult()
+        check(lib().zkgpu_server_poll(self.handle, ticket, wait_ms, ctypes.byref(r)), "zkgpu_server_poll")
+        return r
+
+    def wait(self, ticket):
+        r = self.poll(ticket, -1)
+        check(r.status, "server job")
+        return r
+
+    def release(self, ticket):
+        check(lib().zkgpu_server_release(self.handle, ticket), "zkgpu_server_release")
+
+    def stats(self):
+        st = ServerStats()
+        check(lib().zkgpu_server_stats_get(self.handle, ctypes.byref(st)), "zkgpu_server_stats_get")
+        return {name: getattr(st, name) for name, _ in ServerStats._fields_}
+
+    def close(self):
+        if self.handle:
+            lib().zkgpu_server_destroy(self.handle)
             self.handle = 0
 
     def __del__(self):

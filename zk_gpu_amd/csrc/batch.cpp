@@ -47,31 +47,6 @@ struct Batch {
   }
 };
 
-static int job_from_c(const zkgpu_job& j, JobSpec& out) {
-  if (!j.d_in || !j.d_out || j.log_n < 1 || j.log_n > MAX_LOG_N) return ZK_ERR_INVALID;
-  if (j.op < OP_FWD || j.op > OP_COSET_INV) return ZK_ERR_INVALID;
-  out = JobSpec();
-  out.in = j.d_in;
-  out.out = j.d_out;
-  out.log_n = j.log_n;
-  out.op = j.op;
-  if (j.flags & ZKGPU_JOB_STD_DOMAIN) {
-    const bool inverse = j.op == OP_INV || j.op == OP_COSET_INV;
-    uint64_t w[4];
-    zkgpu_domain_constant(j.log_n, inverse ? 1 : 0, w);
-    out.omega = host::from_ptr(w);
-    zkgpu_domain_constant(j.log_n, 2, w);
-    out.divisor = host::from_ptr(w);
-    zkgpu_domain_constant(j.log_n, j.op == OP_COSET_INV ? 4 : 3, w);
-    out.coset = host::from_ptr(w);
-  } else {
-    out.omega = host::from_ptr(j.omega);
-    out.divisor = host::from_ptr(j.divisor);
-    out.coset = host::from_ptr(j.coset);
-  }
-  return ZK_OK;
-}
-
 int batch_build(int device, const zkgpu_job* cjobs, uint32_t count, Batch** out) {
   *out = nullptr;
   if (!cjobs || count == 0) return ZK_ERR_INVALID;
@@ -85,7 +60,7 @@ int batch_build(int device, const zkgpu_job* cjobs, uint32_t count, Batch** out)
   b->tables.resize(count);
   hipStream_t s = e->thread_stream();
   for (uint32_t i = 0; i < count; ++i) {
-    st = job_from_c(cjobs[i], b->jobs[i]);
+    st = job_from_c(cjobs[i], b->jobs[i], false);
     if (st) return st;
     st = e->prepare(b->jobs[i], b->tables[i], s, PLAN_BATCH);
     if (st) return st;

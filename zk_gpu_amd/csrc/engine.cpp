@@ -1,6 +1,8 @@
 // Per-device NTT engine (see engine.hpp).
 #include "engine.hpp"
 
+#include "../../include/zkgpu_ntt.h"
+
 #include <dlfcn.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -462,6 +464,31 @@ int Engine::build_descs(const JobSpec& job, const JobTables& t, void* ws, PassDe
     descs[i] = d;
   }
   *ndesc = p.npass;
+  return ZK_OK;
+}
+
+int job_from_c(const zkgpu_job& j, JobSpec& out, bool out_optional) {
+  if (!j.d_in || (!j.d_out && !out_optional) || j.log_n < 1 || j.log_n > MAX_LOG_N) return ZK_ERR_INVALID;
+  if (j.op < OP_FWD || j.op > OP_COSET_INV) return ZK_ERR_INVALID;
+  out = JobSpec();
+  out.in = j.d_in;
+  out.out = j.d_out;
+  out.log_n = j.log_n;
+  out.op = j.op;
+  if (j.flags & ZKGPU_JOB_STD_DOMAIN) {
+    const bool inverse = j.op == OP_INV || j.op == OP_COSET_INV;
+    uint64_t w[4];
+    zkgpu_domain_constant(j.log_n, inverse ? 1 : 0, w);
+    out.omega = host::from_ptr(w);
+    zkgpu_domain_constant(j.log_n, 2, w);
+    out.divisor = host::from_ptr(w);
+    zkgpu_domain_constant(j.log_n, j.op == OP_COSET_INV ? 4 : 3, w);
+    out.coset = host::from_ptr(w);
+  } else {
+    out.omega = host::from_ptr(j.omega);
+    out.divisor = host::from_ptr(j.divisor);
+    out.coset = host::from_ptr(j.coset);
+  }
   return ZK_OK;
 }
 

@@ -22,6 +22,8 @@
 #include "host_field.hpp"
 #include "ntt_desc.hpp"
 
+struct zkgpu_job;  // include/zkgpu_ntt.h
+
 namespace zkgpu {
 
 enum Status : int {
@@ -150,6 +152,10 @@ class Engine {
   std::map<std::pair<hipStream_t, int>, std::shared_ptr<DevBuf>> ws_;
   std::string last_error_;
 };
+
+// A C-ABI job as a JobSpec (standard-domain constants filled in when the job
+// asks for them). out_optional: d_out may be NULL (the caller fills it later).
+int job_from_c(const ::zkgpu_job& j, JobSpec& out, bool out_optional);
 
 // HIP launchers (ntt_kernels.hip)
 hipError_t configure_kernels();
