@@ -117,7 +117,7 @@ int zkgpu_check_canonical(int device_id, const void* d_in, uint64_t count, uint3
  * range. Host only. */
 int zkgpu_plan_get(uint32_t log_n, uint32_t* digits, uint32_t* log_cols);
 /* Install a pass split for 2^log_n (the autotuner's choice); returns
- * ZKGPU_ERR_INVALID for an inconsistent split. npass = 0 restores the default. */
+ * ZKGPU_ERR_INVALID for an inconsistent split. npass = 0 drops the installed split (back to the tuned plan file, else the built-in default). */
 int zkgpu_plan_set(uint32_t log_n, uint32_t npass, const uint32_t* digits, const uint32_t* log_cols);
 
 /* Same for the two plan tables: kind 0 = single transforms (tiles up to 2^12

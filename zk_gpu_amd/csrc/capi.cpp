@@ -227,9 +227,7 @@ int zkgpu_plan_set(uint32_t log_n, uint32_t npass, const uint32_t* digits, const
 int zkgpu_plan_set_kind(uint32_t log_n, int kind, uint32_t npass, const uint32_t* digits, const uint32_t* log_cols) {
   if (log_n < 1 || log_n > MAX_LOG_N || kind < 0 || kind > 1) return ZK_ERR_INVALID;
   if (npass == 0) {
-    Plan d;
-    default_plan(log_n, d, plan_tile_log(kind));
-    set_plan_override(d, kind);
+    erase_plan_override(log_n, kind);  // back to the tuned plan file, else the default split
     return ZK_OK;
   }
   if (npass > MAX_PASSES || !digits || !log_cols) return ZK_ERR_INVALID;

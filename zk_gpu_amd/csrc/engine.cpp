@@ -15,7 +15,8 @@ namespace zkgpu {
 // ---------------------------------------------------------------- plans
 
 static std::mutex g_plan_mu;
-static std::map<uint32_t, Plan> g_plan_override[2];
+static std::map<uint32_t, Plan> g_plan_override[2];  // set at run time (autotuner, tests)
+static std::map<uint32_t, Plan> g_plan_file[2];      // tuned plan files
 
 uint32_t plan_tile_log(int kind) { return kind == PLAN_BATCH ? batch_tile_log() : TILE_LOG_MAX; }
 
@@ -102,7 +103,7 @@ static void load_plan_file(int kind) {
       p.digit[i] = v[2 + i];
       p.log_c[i] = v[5 + i];
     }
-    if (plan_valid(p, plan_tile_log(kind))) g_plan_override[kind][p.log_n] = p;
+    if (plan_valid(p, plan_tile_log(kind))) g_plan_file[kind][p.log_n] = p;
   }
   fclose(f);
 }
@@ -123,6 +124,8 @@ Plan plan_for(uint32_t log_n, int kind) {
     std::lock_guard<std::mutex> g(g_plan_mu);
     auto it = g_plan_override[kind].find(log_n);
     if (it != g_plan_override[kind].end()) return it->second;
+    auto f = g_plan_file[kind].find(log_n);
+    if (f != g_plan_file[kind].end()) return f->second;
   }
   Plan p;
   default_plan(log_n, p, plan_tile_log(kind));
@@ -130,12 +133,15 @@ Plan plan_for(uint32_t log_n, int kind) {
 }
 void set_plan_override(const Plan& p, int kind) {
   kind = kind == PLAN_BATCH ? PLAN_BATCH : PLAN_SINGLE;
-  load_plan_files_once();  // the files must not overwrite this override later
   std::lock_guard<std::mutex> g(g_plan_mu);
   g_plan_override[kind][p.log_n] = p;
 }
+void erase_plan_override(uint32_t log_n, int kind) {
+  kind = kind == PLAN_BATCH ? PLAN_BATCH : PLAN_SINGLE;
+  std::lock_guard<std::mutex> g(g_plan_mu);
+  g_plan_override[kind].erase(log_n);
+}
 void clear_plan_overrides() {
-  load_plan_files_once();
   std::lock_guard<std::mutex> g(g_plan_mu);
   g_plan_override[0].clear();
   g_plan_override[1].clear();

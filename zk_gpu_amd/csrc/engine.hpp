@@ -60,7 +60,10 @@ bool default_plan(uint32_t log_n, Plan& out, uint32_t max_tile_log = TILE_LOG_MA
 bool plan_valid(const Plan& p, uint32_t max_tile_log = TILE_LOG_MAX);
 // The plan used for log_n: an installed override (autotuner) or the default.
 Plan plan_for(uint32_t log_n, int kind = PLAN_SINGLE);
+// Plan lookup order: override (set_plan_override), then the tuned plan file,
+// then default_plan.
 void set_plan_override(const Plan& p, int kind = PLAN_SINGLE);
+void erase_plan_override(uint32_t log_n, int kind = PLAN_SINGLE);
 void clear_plan_overrides();
 
 // One transform, device-resident operands.
