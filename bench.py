@@ -19,6 +19,7 @@ the host cores over a bounded sample of the same workload.
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -36,6 +37,39 @@ def modmuls(log_n, op):
     if op == "coset_inv":
         m += n
     return m
+
+
+def profiled_traffic():
+    """HBM bytes per k=20 pass launch (rocprofv3 FETCH_SIZE + WRITE_SIZE,
+    calibrated against a copy of known size), from the committed summary in
+    profiles/; None when it is absent."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "ntt_pass_kernel_k20_traffic.txt")
+    try:
+        with open(path) as f:
+            for line in f:
+                m = re.match(r"TRAFFIC k=20 bytes_per_launch=(\d+)", line)
+                if m:
+                    return int(m.group(1))
+    except OSError:
+        pass
+    return None
+
+
+def bench_host_api(Z, log_n=20, reps=5):
+    """The drop-in host-buffer entry point (parallel_fft_api: host buffer in,
+    transformed in place, copies over the host link included)."""
+    import numpy as np
+    n = 1 << log_n
+    req = np.ascontiguousarray(np.random.default_rng(1).integers(0, 1 << 62, size=(n, 4), dtype=np.uint64))
+    req[:, 3] &= (1 << 60) - 1  # below r
+    omega = Z.domain_constant(log_n, 0)
+    Z.check(Z.parallel_fft_api(0, req, omega, None, n, log_n, 0), "parallel_fft_api")
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        Z.check(Z.parallel_fft_api(0, req, omega, None, n, log_n, 0), "parallel_fft_api")
+    s = (time.perf_counter() - t0) / reps
+    return {"log_n": log_n, "ms": s * 1e3, "elem_per_s": n / s,
+            "note": "host buffer in and out (pageable numpy memory), H2D + transform + D2H per call"}
 
 
 def passes(Z, log_n):
@@ -358,7 +392,7 @@ def main():
                    "log_n": k, "ops": [o for o, _ in ops], "coset_g": 7,
                    "plan": {"digits": digits, "log_cols": cols}},
         "roofline": {"bound": "valu", "kernel": kernel_name, "achieved": achieved, "peak": PEAK_MAD_TOPS,
-                     "unit": "Tmad/s", "frac": achieved / PEAK_MAD_TOPS, "traffic": None,
+                     "unit": "Tmad/s", "frac": achieved / PEAK_MAD_TOPS, "traffic": profiled_traffic(),
                      "launch_ms": launch_ms, "launches_per_step": launches,
                      "work_per_launch": f"{mads_per_launch:.4g} 32x32->64 multiply-adds "
                                         f"({mm_step / launches:.4g} Montgomery products x 136)"},
@@ -399,6 +433,8 @@ def main():
 
     if not args.no_batch:
         res["batched"] = bench_batched(torch, Z, dev, stream)
+
+    res["host_api_k20"] = bench_host_api(Z)
 
     if not args.no_serve:
         res["served"] = bench_serving(torch, Z, dev, stream, count=args.serve_jobs)
