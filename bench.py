@@ -11,8 +11,11 @@ Prints ONE JSON line. `value` is elements transformed per second over the
 timed steps. `roofline` describes the dominant kernel (the NTT pass kernel)
 against the binding bound, the VALU integer multiply rate: algorithmic work
 is (n/2) log2 n Montgomery products per transform (+ n per scaling), each
-136 32x32->64 multiply-adds (8-word CIOS). `roofline_hbm` gives the same
-kernel against HBM bandwidth (64 bytes per element per pass).
+136 32x32->64 multiply-adds (8-word CIOS). `peak` is the nominal rate (one
+v_mad_u64_u32 per lane per clock: 256 CUs x 64 lanes x 2.4 GHz);
+`peak_measured` is the rate a register-only probe of that instruction
+reaches on this device, measured in the same run. `roofline_hbm` gives the
+same kernel against HBM bandwidth (64 bytes per element per pass).
 `cpu_baseline` times the CPU restatement of halo2's best_fft (oracle/) on
 the host cores over a bounded sample of the same workload.
 """
@@ -331,6 +334,7 @@ def main():
     dev = torch.device("cuda", 0)
     stream = torch.cuda.current_stream(dev)
     sptr = stream.cuda_stream
+    p_mul = max(Z.probe_mad_rate() for _ in range(3)) / 1e12  # measured v_mad_u64_u32 rate, T/s
     if args.copy_probe:
         src = torch.ones(1 << 25, dtype=torch.int64, device=dev)
         dst = torch.empty_like(src)
@@ -393,6 +397,7 @@ def main():
                    "plan": {"digits": digits, "log_cols": cols}},
         "roofline": {"bound": "valu", "kernel": kernel_name, "achieved": achieved, "peak": PEAK_MAD_TOPS,
                      "unit": "Tmad/s", "frac": achieved / PEAK_MAD_TOPS, "traffic": profiled_traffic(),
+                     "peak_measured": p_mul, "frac_measured": achieved / p_mul,
                      "launch_ms": launch_ms, "launches_per_step": launches,
                      "work_per_launch": f"{mads_per_launch:.4g} 32x32->64 multiply-adds "
                                         f"({mm_step / launches:.4g} Montgomery products x 136)"},
@@ -425,7 +430,8 @@ def main():
         res["k24"] = {"op": "fwd", "ms": ms24, "elem_per_s": n24 / (ms24 * 1e-3),
                       "plan": {"digits": d24, "log_cols": c24},
                       "roofline": {"bound": "valu", "achieved": ach, "peak": PEAK_MAD_TOPS, "unit": "Tmad/s",
-                                   "frac": ach / PEAK_MAD_TOPS, "launch_ms": l_ms},
+                                   "frac": ach / PEAK_MAD_TOPS, "peak_measured": p_mul, "frac_measured": ach / p_mul,
+                                   "launch_ms": l_ms},
                       "roofline_hbm": {"bound": "hbm", "achieved": 64 * n24 / (l_ms * 1e-3) / 1e9,
                                        "peak": PEAK_HBM_GBS, "unit": "GB/s",
                                        "frac": 64 * n24 / (l_ms * 1e-3) / 1e9 / PEAK_HBM_GBS}}

@@ -160,6 +160,25 @@ int zkgpu_batch_destroy(uint64_t handle);
 /* create + launch + synchronize + destroy. */
 int zkgpu_ntt_batch_device(int device_id, const zkgpu_job* jobs, uint32_t count, void* stream);
 
+/* Extended-domain parts of one polynomial (the caller pattern of the
+ * reference's parallel_coeff_to_extended_part_api, cuda/fft.cu:572-576,
+ * called once per part from src/main.rs:141-148): part j =
+ * FFT_omega(a[i] * cosets[j]^i) for j < count, all parts in one batched
+ * launch per pass level; the input is read from the device and never copied.
+ * cosets holds count elements (4 words each, Montgomery). d_outs[j] must not
+ * alias d_in. Runs on `stream` and returns once the parts are written. */
+#define ZKGPU_MAX_PARTS 64
+int zkgpu_coeff_to_extended_parts_device(int device_id, const void* d_in, void* const* d_outs, uint32_t log_n,
+                                         const uint64_t* omega, const uint64_t* cosets, uint32_t count,
+                                         void* stream);
+/* Same on host buffers: the coefficients are uploaded once, parts (count x n
+ * elements, part j at parts + 4 * n * j) are downloaded together. Replaces
+ * count calls of parallel_coeff_to_extended_part_api, each of which uploads
+ * and downloads the whole polynomial. */
+uint64_t zkgpu_coeff_to_extended_parts_api(size_t device_id, const uint64_t* coeffs, uint64_t* parts,
+                                           const uint64_t* omega, uint64_t n, uint64_t log_n,
+                                           const uint64_t* cosets, uint64_t count);
+
 /* ---------------------------------------------------------------------
  * Serving (replaces the reference's per-call admission, GpuLock in
  * src/gpulock.rs:133-226, for one device): a request loop that takes NTT
@@ -217,6 +236,13 @@ int zkgpu_server_destroy(uint64_t handle);
 /* Resident table bytes on a device; drop them with zkgpu_drop_tables. */
 uint64_t zkgpu_table_bytes(int device_id);
 int zkgpu_drop_tables(int device_id);
+
+/* Measured throughput of the 32x32->64 multiply-add (v_mad_u64_u32) on the
+ * device, in multiply-adds per second: a register-only probe kernel, 8
+ * independent chains per thread, 8 waves per SIMD. The Montgomery products
+ * of every transform kernel are made of this instruction; this is the peak
+ * the VALU roofline is measured against. */
+int zkgpu_probe_mad_rate(int device_id, double* mads_per_s);
 
 /* Text of a return code. */
 const char* zkgpu_error_string(int code);

@@ -62,6 +62,8 @@ SIGNATURES = {
     "zkgpu_batch_info": (_I, [_U64, _P, _P, _P]),
     "zkgpu_batch_destroy": (_I, [_U64]),
     "zkgpu_ntt_batch_device": (_I, [_I, _P, _U32, _P]),
+    "zkgpu_coeff_to_extended_parts_device": (_I, [_I, _P, _P, _U32, _P, _P, _U32, _P]),
+    "zkgpu_coeff_to_extended_parts_api": (_U64, [_SZ, _P, _P, _P, _U64, _U64, _P, _U64]),
     "zkgpu_server_create": (_I, [_I, _U64, _U32, _U32, _P]),
     "zkgpu_server_submit": (_I, [_U64, _P, _P]),
     "zkgpu_server_poll": (_I, [_U64, _U64, ctypes.c_int32, _P]),
@@ -71,6 +73,7 @@ SIGNATURES = {
     "zkgpu_table_bytes": (_U64, [_I]),
     "zkgpu_drop_tables": (_I, [_I]),
     "zkgpu_error_string": (ctypes.c_char_p, [_I]),
+    "zkgpu_probe_mad_rate": (_I, [_I, _P]),
     "zkgpu_version": (_I, []),
 }
 OPTIONAL = {}
@@ -269,6 +272,24 @@ class Batch:
             pass
 
 
+def coeff_to_extended_parts_device(d_in, d_outs, log_n, omega, cosets, stream=None, device_id=0):
+    """part j = FFT_omega(a[i] * cosets[j]^i) on device pointers, one batched run."""
+    outs = (ctypes.c_void_p * len(d_outs))(*d_outs)
+    cs = np.ascontiguousarray(np.asarray(cosets, dtype=np.uint64).reshape(len(d_outs), 4))
+    check(lib().zkgpu_coeff_to_extended_parts_device(device_id, d_in, outs, log_n, _ptr(_elem(omega)), _ptr(cs),
+                                                     len(d_outs), stream), "zkgpu_coeff_to_extended_parts_device")
+
+
+def coeff_to_extended_parts_api(device_id, coeffs, omega, n, log_n, cosets):
+    """Host version: returns a (count, n, 4) uint64 array of parts."""
+    cs = np.ascontiguousarray(np.asarray(cosets, dtype=np.uint64).reshape(-1, 4))
+    parts = np.empty((cs.shape[0], n, 4), dtype=np.uint64)
+    check(lib().zkgpu_coeff_to_extended_parts_api(device_id, _ptr(np.ascontiguousarray(coeffs)), _ptr(parts),
+                                                  _ptr(_elem(omega)), n, log_n, _ptr(cs), cs.shape[0]),
+          "zkgpu_coeff_to_extended_parts_api")
+    return parts
+
+
 # ---------------------------------------------------------------- serving
 
 class ServerResult(ctypes.Structure):
@@ -328,6 +349,13 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def probe_mad_rate(device_id=0):
+    """Measured v_mad_u64_u32 throughput of the device (multiply-adds/s)."""
+    v = ctypes.c_double(0)
+    check(lib().zkgpu_probe_mad_rate(device_id, ctypes.byref(v)), "zkgpu_probe_mad_rate")
+    return v.value
 
 
 def version():

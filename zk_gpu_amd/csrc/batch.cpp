@@ -29,6 +29,9 @@ struct Batch {
   std::vector<JobTables> tables;  // keeps the resident tables referenced
   std::shared_ptr<DevBuf> arena;  // planar workspaces of the multi-pass jobs
   std::shared_ptr<DevBuf> meta;   // descriptors, tile prefix sums, counters
+  void* arena_ptr = nullptr;      // arena->ptr, or an engine workspace
+  void* meta_ptr = nullptr;
+  std::vector<char> host_meta;    // source of the (asynchronous) meta upload
   struct Level {
     uint32_t desc_off, ndesc, total, grid;
   };
@@ -47,7 +50,18 @@ struct Batch {
   }
 };
 
-int batch_build(int device, const zkgpu_job* cjobs, uint32_t count, Batch** out) {
+// capture: record the launches in a hipGraph (zkgpu_batch_*); otherwise the
+// batch is launched directly (batch_launch_direct). ws_stream: take the
+// workspaces and the descriptor buffer from the engine's per-stream buffers
+// of that stream instead of fresh allocations (the batch must then be run
+// and finished on that stream before the stream's next such batch).
+struct BatchOpts {
+  bool capture = true;
+  bool use_ws = false;
+  hipStream_t ws_stream = nullptr;
+};
+
+int batch_build(int device, const zkgpu_job* cjobs, uint32_t count, Batch** out, const BatchOpts& opt) {
   *out = nullptr;
   if (!cjobs || count == 0) return ZK_ERR_INVALID;
   int st = ZK_OK;
@@ -75,7 +89,10 @@ int batch_build(int device, const zkgpu_job* cjobs, uint32_t count, Batch** out)
       arena_bytes += (((size_t)36 << b->jobs[i].log_n) + 255) & ~(size_t)255;
     }
   }
-  if (arena_bytes) {
+  if (arena_bytes && opt.use_ws) {
+    b->arena_ptr = e->workspace(opt.ws_stream, arena_bytes, WS_BATCH_ARENA);
+    if (!b->arena_ptr) return ZK_ERR_NOMEM;
+  } else if (arena_bytes) {
     b->arena = std::make_shared<DevBuf>();
     b->arena->device = device;
     b->arena->bytes = arena_bytes;
@@ -84,13 +101,14 @@ int batch_build(int device, const zkgpu_job* cjobs, uint32_t count, Batch** out)
       b->arena->ptr = nullptr;
       return ZK_ERR_NOMEM;
     }
+    b->arena_ptr = b->arena->ptr;
   }
   // descriptors grouped by level; within a level the biggest jobs first so
   // that the queue ends with short tiles
   std::vector<std::vector<PassDesc>> per_job(count);
   uint32_t max_pass = 0;
   for (uint32_t i = 0; i < count; ++i) {
-    void* ws = ws_off[i] == SIZE_MAX ? nullptr : (char*)b->arena->ptr + ws_off[i];
+    void* ws = ws_off[i] == SIZE_MAX ? nullptr : (char*)b->arena_ptr + ws_off[i];
     PassDesc d[MAX_PASSES];
     uint32_t nd = 0;
     st = e->build_descs(b->jobs[i], b->tables[i], ws, d, &nd);
@@ -129,21 +147,36 @@ int batch_build(int device, const zkgpu_job* cjobs, uint32_t count, Batch** out)
   const size_t cnt_bytes = b->levels.size() * sizeof(uint32_t);
   const size_t ts_off = (desc_bytes + 255) & ~(size_t)255;
   const size_t cnt_off = (ts_off + ts_bytes + 255) & ~(size_t)255;
-  b->meta = std::make_shared<DevBuf>();
-  b->meta->device = device;
-  b->meta->bytes = cnt_off + cnt_bytes;
-  if (hipMalloc(&b->meta->ptr, b->meta->bytes) != hipSuccess) {
-    (void)hipGetLastError();
-    b->meta->ptr = nullptr;
-    return ZK_ERR_NOMEM;
+  const size_t meta_bytes = cnt_off + cnt_bytes;
+  b->host_meta.assign(meta_bytes, 0);
+  memcpy(b->host_meta.data(), all.data(), desc_bytes);
+  memcpy(b->host_meta.data() + ts_off, starts.data(), ts_bytes);
+  if (opt.use_ws) {
+    b->meta_ptr = e->workspace(opt.ws_stream, meta_bytes, WS_BATCH_META);
+    if (!b->meta_ptr) return ZK_ERR_NOMEM;
+    if (hipMemcpyAsync(b->meta_ptr, b->host_meta.data(), meta_bytes, hipMemcpyHostToDevice, opt.ws_stream) !=
+        hipSuccess)
+      return ZK_ERR_HIP;
+  } else {
+    b->meta = std::make_shared<DevBuf>();
+    b->meta->device = device;
+    b->meta->bytes = meta_bytes;
+    if (hipMalloc(&b->meta->ptr, meta_bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      b->meta->ptr = nullptr;
+      return ZK_ERR_NOMEM;
+    }
+    b->meta_ptr = b->meta->ptr;
+    if (hipMemcpy(b->meta_ptr, b->host_meta.data(), meta_bytes, hipMemcpyHostToDevice) != hipSuccess)
+      return ZK_ERR_HIP;
   }
-  std::vector<char> host(b->meta->bytes, 0);
-  memcpy(host.data(), all.data(), desc_bytes);
-  memcpy(host.data() + ts_off, starts.data(), ts_bytes);
-  if (hipMemcpy(b->meta->ptr, host.data(), host.size(), hipMemcpyHostToDevice) != hipSuccess) return ZK_ERR_HIP;
-  b->d_descs = (PassDesc*)b->meta->ptr;
-  b->d_tile_start = (uint32_t*)((char*)b->meta->ptr + ts_off);
-  b->d_counters = (uint32_t*)((char*)b->meta->ptr + cnt_off);
+  b->d_descs = (PassDesc*)b->meta_ptr;
+  b->d_tile_start = (uint32_t*)((char*)b->meta_ptr + ts_off);
+  b->d_counters = (uint32_t*)((char*)b->meta_ptr + cnt_off);
+  if (!opt.capture) {
+    *out = b.release();
+    return ZK_OK;
+  }
 
   // capture: counter reset (a kernel, not a memset node) and one persistent
   // launch per level
@@ -167,6 +200,17 @@ int batch_build(int device, const zkgpu_job* cjobs, uint32_t count, Batch** out)
   return ZK_OK;
 }
 
+// The batch's launches enqueued on s without a graph.
+int batch_launch_direct(const Batch* b, hipStream_t s) {
+  hipError_t le = launch_reset_counters(b->d_counters, (uint32_t)b->levels.size(), s);
+  for (size_t i = 0; le == hipSuccess && i < b->levels.size(); ++i) {
+    const auto& L = b->levels[i];
+    le = launch_persistent(b->d_descs + L.desc_off, b->d_tile_start + L.desc_off, L.ndesc, L.total,
+                           b->d_counters + i, L.grid, s);
+  }
+  return le == hipSuccess ? ZK_OK : ZK_ERR_HIP;
+}
+
 }  // namespace zkgpu
 
 using namespace zkgpu;
@@ -177,7 +221,7 @@ int zkgpu_batch_create(int device_id, const zkgpu_job* jobs, uint32_t count, uin
   if (!handle) return ZK_ERR_INVALID;
   *handle = 0;
   Batch* b = nullptr;
-  const int st = batch_build(device_id, jobs, count, &b);
+  const int st = batch_build(device_id, jobs, count, &b, BatchOpts());
   if (st) return st;
   *handle = (uint64_t)(uintptr_t)b;
   return ZK_OK;
@@ -211,6 +255,61 @@ int zkgpu_batch_destroy(uint64_t handle) {
   if (Engine::get(b->device, &st)) (void)hipDeviceSynchronize();  // queued launches may still read its buffers
   delete b;
   return ZK_OK;
+}
+
+int zkgpu_coeff_to_extended_parts_device(int device_id, const void* d_in, void* const* d_outs, uint32_t log_n,
+                                         const uint64_t* omega, const uint64_t* cosets, uint32_t count,
+                                         void* stream) {
+  if (!d_in || !d_outs || !omega || !cosets || count == 0 || count > ZKGPU_MAX_PARTS) return ZK_ERR_INVALID;
+  if (log_n < 1 || log_n > MAX_LOG_N) return ZK_ERR_INVALID;
+  std::vector<zkgpu_job> jobs(count);
+  for (uint32_t j = 0; j < count; ++j) {
+    if (!d_outs[j] || d_outs[j] == d_in) return ZK_ERR_INVALID;  // the input is read by every part
+    zkgpu_job& jb = jobs[j];
+    memset(&jb, 0, sizeof(jb));
+    jb.d_in = d_in;
+    jb.d_out = d_outs[j];
+    jb.log_n = log_n;
+    jb.op = ZKGPU_OP_COSET_FWD;
+    memcpy(jb.omega, omega, 32);
+    memcpy(jb.coset, cosets + 4 * (size_t)j, 32);
+  }
+  hipStream_t s = (hipStream_t)stream;
+  BatchOpts opt;
+  opt.capture = false;
+  opt.use_ws = true;
+  opt.ws_stream = s;
+  Batch* b = nullptr;
+  int st = batch_build(device_id, jobs.data(), count, &b, opt);
+  if (st == ZK_OK) st = batch_launch_direct(b, s);
+  // the batch's descriptors and workspaces are the stream's: finish here
+  if (hipStreamSynchronize(s) != hipSuccess && st == ZK_OK) st = ZK_ERR_HIP;
+  delete b;
+  return st;
+}
+
+uint64_t zkgpu_coeff_to_extended_parts_api(size_t device_id, const uint64_t* coeffs, uint64_t* parts,
+                                           const uint64_t* omega, uint64_t n, uint64_t log_n,
+                                           const uint64_t* cosets, uint64_t count) {
+  if (!coeffs || !parts || !omega || !cosets || count == 0 || count > ZKGPU_MAX_PARTS) return ZK_ERR_INVALID;
+  if (log_n < 1 || log_n > MAX_LOG_N || n != (1ull << log_n)) return ZK_ERR_INVALID;
+  int st = ZK_OK;
+  Engine* e = Engine::get((int)device_id, &st);
+  if (!e) return (uint64_t)st;
+  hipStream_t s = e->thread_stream();
+  if (!s) return ZK_ERR_HIP;
+  const size_t bytes = (size_t)32 << log_n;
+  void* d_in = e->workspace(s, bytes, WS_PARTS_IN);
+  void* d_out = e->workspace(s, bytes * count, WS_PARTS_OUT);
+  if (!d_in || !d_out) return ZK_ERR_NOMEM;
+  std::vector<void*> outs(count);
+  for (uint64_t j = 0; j < count; ++j) outs[j] = (char*)d_out + j * bytes;
+  if (hipMemcpyAsync(d_in, coeffs, bytes, hipMemcpyHostToDevice, s) != hipSuccess) return ZK_ERR_HIP;
+  st = zkgpu_coeff_to_extended_parts_device((int)device_id, d_in, outs.data(), (uint32_t)log_n, omega, cosets,
+                                            (uint32_t)count, s);
+  if (st) return (uint64_t)st;
+  if (hipMemcpyAsync(parts, d_out, bytes * count, hipMemcpyDeviceToHost, s) != hipSuccess) return ZK_ERR_HIP;
+  return hipStreamSynchronize(s) == hipSuccess ? ZK_OK : ZK_ERR_HIP;
 }
 
 int zkgpu_ntt_batch_device(int device_id, const zkgpu_job* jobs, uint32_t count, void* stream) {

@@ -27,7 +27,7 @@ int host_call(size_t device_id, uint64_t* req, uint64_t log_n, int op, const uin
   hipStream_t s = e->thread_stream();
   if (!s) return ZK_ERR_HIP;
   const size_t bytes = (size_t)32 << log_n;
-  void* d = e->workspace(s, bytes, 1);
+  void* d = e->workspace(s, bytes, WS_HOST_IO);
   if (!d) return ZK_ERR_NOMEM;
   JobSpec job;
   job.in = d;
@@ -81,7 +81,7 @@ uint64_t do_fft_ffi(size_t device_id) {
   hipStream_t s = e->thread_stream();
   const uint32_t k = 10;
   const size_t bytes = (size_t)32 << k;
-  void* a = e->workspace(s, 2 * bytes, 2);
+  void* a = e->workspace(s, 2 * bytes, WS_SMOKE);
   if (!a) return ZK_ERR_NOMEM;
   void* b = (char*)a + bytes;
   if (launch_fill_random(a, 1, 0, 1ull << k, s) != hipSuccess) return ZK_ERR_HIP;
@@ -269,6 +269,44 @@ const char* zkgpu_error_string(int code) {
     case ZK_ERR_SHUTDOWN: return "server stopping";
     default: return "unknown error";
   }
+}
+
+int zkgpu_probe_mad_rate(int device_id, double* mads_per_s) {
+  if (!mads_per_s) return ZK_ERR_INVALID;
+  *mads_per_s = 0;
+  int st = ZK_OK;
+  if (!Engine::get(device_id, &st)) return st;
+  int cus = 256;
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_id);
+  const uint32_t blocks = (uint32_t)cus * 8;  // 8 waves per SIMD
+  uint64_t* out = nullptr;
+  if (hipMalloc(&out, (size_t)blocks * 256 * sizeof(uint64_t)) != hipSuccess) {
+    (void)hipGetLastError();
+    return ZK_ERR_NOMEM;
+  }
+  hipStream_t s = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = launch_mad_probe(out, blocks, 1, s);  // warm-up
+  if (e == hipSuccess) e = hipEventRecord(e0, s);
+  if (e == hipSuccess) e = launch_mad_probe(out, blocks, 2, s);
+  if (e == hipSuccess) e = hipEventRecord(e1, s);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float ms = 0;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  if (e == hipSuccess && ms > 0)
+    *mads_per_s = (double)blocks * 256.0 * (double)mad_probe_ops_per_thread() / (ms * 1e-3);
+  if (e1) (void)hipEventDestroy(e1);
+  if (e0) (void)hipEventDestroy(e0);
+  if (s) (void)hipStreamDestroy(s);
+  (void)hipFree(out);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return ZK_ERR_HIP;
+  }
+  return ZK_OK;
 }
 
 int zkgpu_version(void) { return 100; }

@@ -506,7 +506,7 @@ int Engine::run(const JobSpec& job, hipStream_t s) {
   if (st) return st;
   void* ws = nullptr;
   if (t.plan.npass > 1) {
-    ws = workspace(s, (size_t)36 << job.log_n, 0);  // planar limb form
+    ws = workspace(s, (size_t)36 << job.log_n, WS_PLANAR);  // planar limb form
     if (!ws) return ZK_ERR_NOMEM;
   }
   PassDesc descs[MAX_PASSES];

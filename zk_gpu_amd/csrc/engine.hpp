@@ -115,6 +115,17 @@ struct JobTables {
   uint32_t scale261[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 };
 
+// Per-stream workspace slots (Engine::workspace).
+enum WsSlot : int {
+  WS_PLANAR = 0,       // inter-pass workspace of a single transform
+  WS_HOST_IO = 1,      // device copy of a host-buffer call
+  WS_SMOKE = 2,        // do_fft_ffi
+  WS_BATCH_ARENA = 3,  // workspaces of an uncaptured batch
+  WS_BATCH_META = 4,   // descriptors and counters of an uncaptured batch
+  WS_PARTS_IN = 5,     // host-buffer extended parts: input
+  WS_PARTS_OUT = 6,    // host-buffer extended parts: outputs
+};
+
 class Engine {
  public:
   static Engine* get(int device, int* status);
@@ -129,7 +140,8 @@ class Engine {
   // Enqueue one transform on `s` (asynchronous).
   int run(const JobSpec& job, hipStream_t s);
 
-  // Grow-only buffers owned per stream (reused by every call on that stream).
+  // Grow-only buffers owned per stream (reused by every call on that
+  // stream), one per WsSlot.
   void* workspace(hipStream_t s, size_t bytes, int slot);
   hipStream_t thread_stream();
 
@@ -175,6 +187,8 @@ hipError_t launch_persistent(const PassDesc* d_descs, const uint32_t* d_tile_sta
                              uint32_t* d_counter, uint32_t grid, hipStream_t s);
 hipError_t launch_reset_counters(uint32_t* d_counter, uint32_t n, hipStream_t s);
 uint32_t batch_tile_log();
+hipError_t launch_mad_probe(uint64_t* out, uint32_t blocks, uint32_t seed, hipStream_t s);
+uint64_t mad_probe_ops_per_thread();
 uint32_t tile_threads(uint32_t log_n_tile, uint32_t log_c);
 uint32_t tile_lds_bytes(uint32_t log_n_tile, uint32_t log_c);
 

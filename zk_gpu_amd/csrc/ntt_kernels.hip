@@ -655,6 +655,35 @@ hipError_t launch_check_canonical(const void* in, uint64_t count, uint32_t* flag
   return hipGetLastError();
 }
 
+// Throughput probe of the 32x32->64 multiply-add (v_mad_u64_u32), the
+// instruction the Montgomery products are made of: 8 independent chains per
+// thread, each step feeding the accumulator's high word back as a factor so
+// that nothing can be folded; registers only, one store per thread.
+constexpr int MAD_PROBE_CHAINS = 8;
+constexpr int MAD_PROBE_ITERS = 2048;
+__global__ void __launch_bounds__(256) mad_probe_kernel(uint64_t* out, uint32_t seed) {
+  uint64_t acc[MAD_PROBE_CHAINS];
+  uint32_t mulv[MAD_PROBE_CHAINS];
+#pragma unroll
+  for (int c = 0; c < MAD_PROBE_CHAINS; ++c) {
+    acc[c] = seed + threadIdx.x * 977u + c;
+    mulv[c] = 0x9E3779B9u * (c + 1) + seed;
+  }
+  for (int it = 0; it < MAD_PROBE_ITERS; ++it) {
+#pragma unroll
+    for (int c = 0; c < MAD_PROBE_CHAINS; ++c) acc[c] = (uint64_t)(uint32_t)(acc[c] >> 32) * mulv[c] + acc[c];
+  }
+  uint64_t x = 0;
+#pragma unroll
+  for (int c = 0; c < MAD_PROBE_CHAINS; ++c) x ^= acc[c];
+  out[blockIdx.x * blockDim.x + threadIdx.x] = x;
+}
+uint64_t mad_probe_ops_per_thread() { return (uint64_t)MAD_PROBE_CHAINS * MAD_PROBE_ITERS; }
+hipError_t launch_mad_probe(uint64_t* out, uint32_t blocks, uint32_t seed, hipStream_t s) {
+  mad_probe_kernel<<<blocks, 256, 0, s>>>(out, seed);
+  return hipGetLastError();
+}
+
 hipError_t configure_kernels() {
   const int max_lds = (int)(4u * NLIMBS) << TILE_LOG_MAX;
   hipError_t e = hipSuccess;
