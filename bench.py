@@ -160,6 +160,72 @@ def bench_batched(torch, Z, dev, stream, count=64, seed=4, reps=10):
             "speedup_vs_plain_sync": plain_sync_ms / batch_ms, "speedup_vs_plain_async": plain_async_ms / batch_ms}
 
 
+R_MOD = 0x30644E72E131A029B85045B68181585D2833E84879B9709143E1F593F0000001
+
+
+def _std_cosets(Z, log_n, parts):
+    """halo2's extended-domain coset shifts c_j = 7 * omega_ext^j (omega_ext of
+    order 2^(log_n + log2 parts)), Montgomery words."""
+    import numpy as np
+    e = parts.bit_length() - 1
+    rm = (1 << 256) % R_MOD
+    w = Z.domain_constant(log_n + e, 0)
+    wv = sum(int(x) << (64 * i) for i, x in enumerate(w)) * pow(rm, -1, R_MOD) % R_MOD
+    out = np.zeros((parts, 4), dtype=np.uint64)
+    for j in range(parts):
+        v = 7 * pow(wv, j, R_MOD) % R_MOD * rm % R_MOD
+        out[j] = [(v >> (64 * i)) & ((1 << 64) - 1) for i in range(4)]
+    return out
+
+
+def bench_parts(torch, Z, dev, stream, log_n=20, parts=4, reps=5):
+    """Extended-domain parts of one 2^log_n polynomial: one batched call
+    against `parts` separate coset-forward transforms (device-resident), and
+    the host-buffer entry point against `parts` calls of
+    parallel_coeff_to_extended_part_api (each uploads and downloads)."""
+    import numpy as np
+    n = 1 << log_n
+    sptr = stream.cuda_stream
+    cs = _std_cosets(Z, log_n, parts)
+    w = Z.domain_constant(log_n, 0)
+    x = torch.empty(n * 4, dtype=torch.int64, device=dev)
+    Z.fill_random(x.data_ptr(), n, 4242, stream=sptr)
+    outs = [torch.empty_like(x) for _ in range(parts)]
+    torch.cuda.synchronize()
+
+    def batched():
+        Z.coeff_to_extended_parts_device(x.data_ptr(), [o.data_ptr() for o in outs], log_n, w, cs, stream=sptr)
+
+    def separate():
+        for j, o in enumerate(outs):
+            Z.ntt_device(x.data_ptr(), o.data_ptr(), log_n, Z.OP_COSET_FWD, w, None, cs[j], stream=sptr)
+        stream.synchronize()
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps * 1e3
+
+    res = {"log_n": log_n, "parts": parts, "device_batched_ms": timed(batched), "device_separate_ms": timed(separate)}
+    h = np.ascontiguousarray(x.cpu().numpy().view(np.uint64).reshape(n, 4))
+    hout = np.zeros((parts, n, 4), dtype=np.uint64)
+    Z.coeff_to_extended_parts_api(0, h, w, n, log_n, cs, out=hout)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        Z.coeff_to_extended_parts_api(0, h, w, n, log_n, cs, out=hout)
+    res["host_batched_ms"] = (time.perf_counter() - t0) / reps * 1e3
+    copies = [h.copy() for _ in range(parts * reps)]
+    t0 = time.perf_counter()
+    for i, c in enumerate(copies):
+        Z.check(Z.parallel_coeff_to_extended_part_api(0, c, w, None, n, log_n, 0, cs[i % parts]))
+    res["host_separate_ms"] = (time.perf_counter() - t0) / reps * 1e3
+    return res
+
+
 def _serve_spec(count, seed=5, lo=14, hi=22):
     import random
     rnd = random.Random(seed)
@@ -441,6 +507,7 @@ def main():
         res["batched"] = bench_batched(torch, Z, dev, stream)
 
     res["host_api_k20"] = bench_host_api(Z)
+    res["extended_parts"] = bench_parts(torch, Z, dev, stream)
 
     if not args.no_serve:
         res["served"] = bench_serving(torch, Z, dev, stream, count=args.serve_jobs)

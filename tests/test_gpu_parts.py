@@ -33,9 +33,9 @@ def test_parts_device_vs_oracle(torch_cuda, k, parts):
     assert (to_host(d_in) == x).all()  # the input is left as it was
 
 
-def test_parts_device_vs_single_calls(torch_cuda):
+@pytest.mark.parametrize("k,parts", [(17, 8), (19, 4)])  # batched / one transform per part
+def test_parts_device_vs_single_calls(torch_cuda, k, parts):
     torch = torch_cuda
-    k, parts = 19, 4
     n = 1 << k
     d_in = dev_empty(torch, n)
     Z.fill_random(ptr(d_in), n, 31)

@@ -280,10 +280,12 @@ def coeff_to_extended_parts_device(d_in, d_outs, log_n, omega, cosets, stream=No
                                                      len(d_outs), stream), "zkgpu_coeff_to_extended_parts_device")
 
 
-def coeff_to_extended_parts_api(device_id, coeffs, omega, n, log_n, cosets):
-    """Host version: returns a (count, n, 4) uint64 array of parts."""
+def coeff_to_extended_parts_api(device_id, coeffs, omega, n, log_n, cosets, out=None):
+    """Host version: returns a (count, n, 4) uint64 array of parts (written
+    into `out` when given)."""
     cs = np.ascontiguousarray(np.asarray(cosets, dtype=np.uint64).reshape(-1, 4))
-    parts = np.empty((cs.shape[0], n, 4), dtype=np.uint64)
+    parts = np.empty((cs.shape[0], n, 4), dtype=np.uint64) if out is None else out
+    assert parts.shape == (cs.shape[0], n, 4) and parts.dtype == np.uint64 and parts.flags["C_CONTIGUOUS"]
     check(lib().zkgpu_coeff_to_extended_parts_api(device_id, _ptr(np.ascontiguousarray(coeffs)), _ptr(parts),
                                                   _ptr(_elem(omega)), n, log_n, _ptr(cs), cs.shape[0]),
           "zkgpu_coeff_to_extended_parts_api")

@@ -200,6 +200,10 @@ int batch_build(int device, const zkgpu_job* cjobs, uint32_t count, Batch** out,
   return ZK_OK;
 }
 
+// Extended-domain parts of at least 2^19 elements run one after another as
+// single transforms; smaller ones share persistent-kernel launches.
+constexpr uint32_t kPartsAloneLog = 19;
+
 // The batch's launches enqueued on s without a graph.
 int batch_launch_direct(const Batch* b, hipStream_t s) {
   hipError_t le = launch_reset_counters(b->d_counters, (uint32_t)b->levels.size(), s);
@@ -275,6 +279,20 @@ int zkgpu_coeff_to_extended_parts_device(int device_id, const void* d_in, void* 
     memcpy(jb.coset, cosets + 4 * (size_t)j, 32);
   }
   hipStream_t s = (hipStream_t)stream;
+  if (log_n >= kPartsAloneLog) {
+    // each part fills the device on its own: the single-transform plans
+    // (tiles up to 2^12 elements, fewer passes) run them faster than a batch
+    int st = ZK_OK;
+    Engine* e = Engine::get(device_id, &st);
+    if (!e) return st;
+    for (uint32_t j = 0; j < count && st == ZK_OK; ++j) {
+      JobSpec job;
+      st = job_from_c(jobs[j], job, false);
+      if (st == ZK_OK) st = e->run(job, s);
+    }
+    if (hipStreamSynchronize(s) != hipSuccess && st == ZK_OK) st = ZK_ERR_HIP;
+    return st;
+  }
   BatchOpts opt;
   opt.capture = false;
   opt.use_ws = true;
@@ -304,12 +322,12 @@ uint64_t zkgpu_coeff_to_extended_parts_api(size_t device_id, const uint64_t* coe
   if (!d_in || !d_out) return ZK_ERR_NOMEM;
   std::vector<void*> outs(count);
   for (uint64_t j = 0; j < count; ++j) outs[j] = (char*)d_out + j * bytes;
-  if (hipMemcpyAsync(d_in, coeffs, bytes, hipMemcpyHostToDevice, s) != hipSuccess) return ZK_ERR_HIP;
-  st = zkgpu_coeff_to_extended_parts_device((int)device_id, d_in, outs.data(), (uint32_t)log_n, omega, cosets,
-                                            (uint32_t)count, s);
+  st = stage_h2d((int)device_id, s, d_in, coeffs, bytes);
+  if (st == ZK_OK)
+    st = zkgpu_coeff_to_extended_parts_device((int)device_id, d_in, outs.data(), (uint32_t)log_n, omega, cosets,
+                                              (uint32_t)count, s);
   if (st) return (uint64_t)st;
-  if (hipMemcpyAsync(parts, d_out, bytes * count, hipMemcpyDeviceToHost, s) != hipSuccess) return ZK_ERR_HIP;
-  return hipStreamSynchronize(s) == hipSuccess ? ZK_OK : ZK_ERR_HIP;
+  return (uint64_t)stage_d2h((int)device_id, s, parts, d_out, bytes * count);
 }
 
 int zkgpu_ntt_batch_device(int device_id, const zkgpu_job* jobs, uint32_t count, void* stream) {

@@ -37,15 +37,13 @@ int host_call(size_t device_id, uint64_t* req, uint64_t log_n, int op, const uin
   job.omega = host::from_ptr(omega);
   if (divisor) job.divisor = host::from_ptr(divisor);
   if (c) job.coset = host::from_ptr(c);
-  if (hipMemcpyAsync(d, req, bytes, hipMemcpyHostToDevice, s) != hipSuccess) return ZK_ERR_HIP;
-  st = e->run(job, s);
+  st = stage_h2d((int)device_id, s, d, req, bytes);
+  if (st == ZK_OK) st = e->run(job, s);
   if (st) {
     (void)hipStreamSynchronize(s);
     return st;
   }
-  if (hipMemcpyAsync(req, d, bytes, hipMemcpyDeviceToHost, s) != hipSuccess) return ZK_ERR_HIP;
-  if (hipStreamSynchronize(s) != hipSuccess) return ZK_ERR_HIP;
-  return ZK_OK;
+  return stage_d2h((int)device_id, s, req, d, bytes);
 }
 
 // Standard-domain constants (omega, omega^-1, n^-1, g, g^-1) for every

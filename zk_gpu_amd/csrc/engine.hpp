@@ -172,6 +172,12 @@ class Engine {
 // asks for them). out_optional: d_out may be NULL (the caller fills it later).
 int job_from_c(const ::zkgpu_job& j, JobSpec& out, bool out_optional);
 
+// Copies between caller (pageable) host memory and the device through pinned
+// staging, chunked and overlapped with host-side copies (host_stage.cpp).
+// stage_h2d enqueues on s; stage_d2h returns once the data is in dst.
+int stage_h2d(int device, hipStream_t s, void* dst, const void* src, size_t bytes);
+int stage_d2h(int device, hipStream_t s, void* dst, const void* src, size_t bytes);
+
 // HIP launchers (ntt_kernels.hip)
 hipError_t configure_kernels();
 hipError_t launch_pass(const PassDesc& d, hipStream_t s);
