@@ -108,3 +108,44 @@ def test_serve_destroy_with_jobs_in_flight(torch_cuda):
         srv.submit(Z.make_job(ptr(x), ptr(x), 16, Z.OP_FWD))
     srv.close()  # admitted jobs finish, pending ones are cancelled; must not hang
     torch.cuda.synchronize()
+
+
+def test_serve_concurrent_submitters(torch_cuda):
+    """Four host threads submit and wait at once; every result equals the job
+    run alone."""
+    import threading
+    torch = torch_cuda
+    rnd = random.Random(21)
+    specs = [[(rnd.randint(10, 17), rnd.choice(OPNAMES)) for _ in range(10)] for _ in range(4)]
+    ins = [[dev_empty(torch, 1 << k) for k, _ in sp] for sp in specs]
+    for t, sp in enumerate(specs):
+        for i, (k, _) in enumerate(sp):
+            Z.fill_random(ptr(ins[t][i]), 1 << k, 60_000 + 100 * t + i)
+    outs = [[dev_empty(torch, 1 << k) for k, _ in sp] for sp in specs]
+    torch.cuda.synchronize()
+    srv = Z.Server()
+    errs = []
+
+    def worker(t):
+        try:
+            tickets = [srv.submit(Z.make_job(ptr(x), ptr(o), k, Z.OPS[op]))
+                       for (k, op), x, o in zip(specs[t], ins[t], outs[t])]
+            for tk in tickets:
+                srv.wait(tk)
+                srv.release(tk)
+        except Exception as e:  # reported below
+            errs.append((t, repr(e)))
+
+    try:
+        ts = [threading.Thread(target=worker, args=(t,)) for t in range(4)]
+        for th in ts:
+            th.start()
+        for th in ts:
+            th.join()
+        assert not errs, errs
+        for t, sp in enumerate(specs):
+            for (k, op), x, o in zip(sp, ins[t], outs[t]):
+                assert (to_host(o) == _alone(torch, x, k, op)).all(), (t, k, op)
+        assert srv.stats()["completed"] == 40
+    finally:
+        srv.close()

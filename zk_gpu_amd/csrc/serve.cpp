@@ -17,8 +17,10 @@
 // Up to kDepth iterations are in flight, so the device never waits for the
 // host between iterations; a job submitted meanwhile joins at the next
 // iteration issued. Per-iteration descriptors go through pinned staging (one
-// host-to-device copy, which also zeroes the tile counter), so an iteration is
-// one copy plus one kernel.
+// host-to-device copy of the used part, which also zeroes the tile counter),
+// so an iteration is one copy plus one kernel; an iteration that holds a
+// single job launches its pass as a pass kernel instead, with the descriptor
+// as the kernel argument.
 #include <hip/hip_runtime.h>
 #include <string.h>
 
@@ -43,6 +45,7 @@ namespace {
 constexpr int kDepth = 2;                         // iterations in flight
 constexpr uint64_t kDefaultBudget = 64ull << 30;  // 64 GiB
 constexpr uint32_t kDefaultMaxActive = 256;
+constexpr uint64_t kIdleSpinNs = 200000;  // idle scheduler spins this long before sleeping
 
 uint64_t now_ns() {
   return (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
@@ -139,10 +142,10 @@ class Server {
     int cus = 256;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_);
     grid_cap_ = (uint32_t)cus * 2;  // two resident 2048-element tiles per CU
-    desc_off_ = 0;
-    ts_off_ = round256((size_t)max_active_ * sizeof(PassDesc));
-    cnt_off_ = round256(ts_off_ + (size_t)max_active_ * sizeof(uint32_t));
-    stage_bytes_ = cnt_off_ + 256;
+    // staging layout of an iteration: counter | tile starts | descriptors;
+    // only the used part of it is copied
+    ts_off_ = 256;
+    stage_bytes_ = ts_off_ + round256((size_t)max_active_ * sizeof(uint32_t)) + (size_t)max_active_ * sizeof(PassDesc);
     for (auto& it : ring_) {
       if (hipEventCreateWithFlags(&it.done, hipEventDisableTiming) != hipSuccess) return ZK_ERR_HIP;
       if (hipHostMalloc(&it.host, stage_bytes_, hipHostMallocDefault) != hipSuccess) return ZK_ERR_NOMEM;
@@ -158,6 +161,7 @@ class Server {
       {
         std::lock_guard<std::mutex> lk(mu_);
         stop_ = true;
+        ++gen_;
       }
       cv_work_.notify_all();
       thread_.join();
@@ -310,9 +314,10 @@ class Server {
     // biggest transforms first so that the tile queue ends with short tiles
     std::stable_sort(order.begin(), order.end(),
                      [](const SJob* a, const SJob* b) { return a->cjob.log_n > b->cjob.log_n; });
-    PassDesc* hd = (PassDesc*)((char*)it.host + desc_off_);
+    const size_t desc_off = ts_off_ + round256(order.size() * sizeof(uint32_t));
+    PassDesc* hd = (PassDesc*)((char*)it.host + desc_off);
     uint32_t* hts = (uint32_t*)((char*)it.host + ts_off_);
-    uint32_t* hcnt = (uint32_t*)((char*)it.host + cnt_off_);
+    uint32_t* hcnt = (uint32_t*)it.host;
     uint32_t nd = 0, total = 0;
     const uint64_t t = now_ns();
     for (SJob* j : order) {
@@ -330,10 +335,16 @@ class Server {
     active_.erase(std::remove_if(active_.begin(), active_.end(), [](SJob* j) { return j->next == j->npass; }),
                   active_.end());
     finishing_count_ += it.finishing.size();
-    hipError_t e = hipMemcpyAsync(it.dev, it.host, stage_bytes_, hipMemcpyHostToDevice, stream_);
-    if (e == hipSuccess) {
-      e = launch_persistent((const PassDesc*)((char*)it.dev + desc_off_), (const uint32_t*)((char*)it.dev + ts_off_),
-                            nd, total, (uint32_t*)((char*)it.dev + cnt_off_), std::min(total, grid_cap_), stream_);
+    hipError_t e = hipSuccess;
+    if (nd == 1) {
+      // one job in the iteration: its pass as a pass-kernel launch (the
+      // descriptor travels as the kernel argument; no copy, no counter)
+      e = launch_pass(hd[0], stream_);
+    } else {
+      e = hipMemcpyAsync(it.dev, it.host, desc_off + nd * sizeof(PassDesc), hipMemcpyHostToDevice, stream_);
+      if (e == hipSuccess)
+        e = launch_persistent((const PassDesc*)((char*)it.dev + desc_off), (const uint32_t*)((char*)it.dev + ts_off_),
+                              nd, total, (uint32_t*)it.dev, std::min(total, grid_cap_), stream_);
     }
     if (e == hipSuccess) e = hipEventRecord(it.done, stream_);
     if (e != hipSuccess) {
@@ -434,10 +445,16 @@ class Server {
         }
         lk.lock();
       } else {
-        // idle, or pending jobs that do not fit yet: sleep until a
-        // submission, a release or shutdown changes that
-        const uint64_t g = gen_;
-        cv_work_.wait(lk, [&] { return stop_ || gen_ != g; });
+        // idle, or pending jobs that do not fit yet: wait until a
+        // submission, a release or shutdown changes that -- spinning for a
+        // short while first, so that a job arriving soon is not delayed by a
+        // thread wake-up, then sleeping
+        const uint64_t g = gen_.load(std::memory_order_relaxed);
+        lk.unlock();
+        const uint64_t t_end = now_ns() + kIdleSpinNs;
+        while (gen_.load(std::memory_order_acquire) == g && now_ns() < t_end) std::this_thread::yield();
+        lk.lock();
+        cv_work_.wait(lk, [&] { return stop_ || gen_.load(std::memory_order_relaxed) != g; });
       }
     }
   }
@@ -449,7 +466,7 @@ class Server {
   Pool pool_;
   hipStream_t stream_ = nullptr;
   uint32_t grid_cap_ = 512;
-  size_t desc_off_ = 0, ts_off_ = 0, cnt_off_ = 0, stage_bytes_ = 0;
+  size_t ts_off_ = 0, stage_bytes_ = 0;
   Iter ring_[kDepth];
   int head_ = 0, inflight_ = 0;
 
@@ -458,7 +475,7 @@ class Server {
   std::atomic<uint64_t> submissions_{0};
   std::thread thread_;
   bool stop_ = false;
-  uint64_t gen_ = 0;  // bumped by submit, release and shutdown
+  std::atomic<uint64_t> gen_{0};  // bumped (under mu_) by submit, release and shutdown
   uint64_t next_ticket_ = 0;
   std::deque<SJob*> pending_;
   std::vector<SJob*> active_;  // admitted, passes left to issue
