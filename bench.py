@@ -102,9 +102,15 @@ def cpu_baseline(budget_s=15.0):
         if el > budget_s or steps >= 20:
             break
     value = steps * len(ops) * (1 << k) / el
+    # one thread (log_split = 0: halo2's serial path), one step
+    t0 = time.perf_counter()
+    for op in ops:
+        O.run_op(op, x, k, threads=1)
+    value_1 = len(ops) * (1 << k) / (time.perf_counter() - t0)
     return {"value": value, "unit": "elem/s", "cores": cores, "kind": "port",
             "sample": f"{steps} steps of fwd+inv+coset_fwd at 2^20 (halo2 best_fft restated in C, "
-                      f"log_split=floor(log2 {cores}))"}
+                      f"log_split=floor(log2 {cores}))",
+            "value_1_thread": value_1, "sample_1_thread": "1 step of fwd+inv+coset_fwd at 2^20, 1 thread"}
 
 
 def bench_batched(torch, Z, dev, stream, count=64, seed=4, reps=10):
@@ -384,6 +390,8 @@ def main():
     ap.add_argument("--no-k24", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-host", action="store_true", help="skip the host-buffer entry point timing")
+    ap.add_argument("--no-parts", action="store_true", help="skip the extended-domain parts timing")
     ap.add_argument("--serve-jobs", type=int, default=10_000)
     ap.add_argument("--copy-probe", action="store_true",
                     help="copy 256 MiB once on the device first: a kernel of known traffic for "
@@ -506,8 +514,10 @@ def main():
     if not args.no_batch:
         res["batched"] = bench_batched(torch, Z, dev, stream)
 
-    res["host_api_k20"] = bench_host_api(Z)
-    res["extended_parts"] = bench_parts(torch, Z, dev, stream)
+    if not args.no_host:
+        res["host_api_k20"] = bench_host_api(Z)
+    if not args.no_parts:
+        res["extended_parts"] = bench_parts(torch, Z, dev, stream)
 
     if not args.no_serve:
         res["served"] = bench_serving(torch, Z, dev, stream, count=args.serve_jobs)
