@@ -43,3 +43,25 @@ def test_all_candidate_splits(torch_cuda, k, kind):
         Z.plan_reset(k, kind=plan_kind)
         Z.lib().zkgpu_drop_tables(0)
     assert not bad, bad
+
+
+@pytest.mark.parametrize("k", list(range(12, 24)))
+def test_tuned_plans_vs_oracle(torch_cuda, k):
+    """The split each plan file holds for 2^k (whatever the tuner chose),
+    single-transform table through the pass kernels and batch table through
+    the persistent kernel, against the oracle."""
+    torch = torch_cuda
+    n = 1 << k
+    x = O.gen(1300 + k, n)
+    d_in = to_dev(torch, x)
+    d_out = dev_empty(torch, n)
+    for op in ("fwd", "coset_inv"):
+        want = to_dev(torch, O.run_op(op, x, k, threads=8))
+        Z.ntt_std_device(ptr(d_in), ptr(d_out), k, Z.OPS[op])
+        torch.cuda.synchronize()
+        assert torch.equal(d_out, want), ("single", op, Z.plan_get(k, Z.PLAN_SINGLE))
+        if k <= 22:
+            srv = Z.Server()
+            srv.wait(srv.submit(Z.make_job(ptr(d_in), ptr(d_out), k, Z.OPS[op])))
+            srv.close()
+            assert torch.equal(d_out, want), ("batch", op, Z.plan_get(k, Z.PLAN_BATCH))

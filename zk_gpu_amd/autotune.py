@@ -29,10 +29,11 @@ DEFAULT_OUT = {"single": os.path.join(HERE, "plans_gfx950.txt"),
 TILE_LOG = {"single": 12, "batch": 11}
 
 
-def candidates(k, min_tiles_log=8, max_passes=3, tile_log_max=12):
+def candidates(k, min_tiles_log=8, max_passes=3, tile_log_max=12, wide=False):
     """Valid pass splits of 2^k: digits each <= 12, tile N*C <= 4096,
     column limits as in the engine (pass p < P: columns <= 2^(remaining),
-    last pass: columns <= 2^digit0)."""
+    last pass: columns <= 2^digit0). By default every pass aims at one
+    common tile size; wide=True lets each pass choose its own."""
     out = []
     TILE_LOG_MAX = tile_log_max
     for npass in range(1, max_passes + 1):
@@ -61,8 +62,14 @@ def candidates(k, min_tiles_log=8, max_passes=3, tile_log_max=12):
                 avails.append(min(avail, TILE_LOG_MAX - d))
                 log_m -= d
             seen = set()
-            for target in [t for t in (0, 8, 9, 10, 11, 12) if t <= TILE_LOG_MAX]:
-                cols = tuple(min(max(target - d, 0), a) for d, a in zip(digits, avails))
+            targets = [t for t in (0, 8, 9, 10, 11, 12) if t <= TILE_LOG_MAX]
+            if wide:
+                per_pass = [sorted(set(min(max(t - d, 0), a) for t in targets)) for d, a in zip(digits, avails)]
+                col_sets = itertools.product(*per_pass)
+            else:
+                col_sets = (tuple(min(max(t - d, 0), a) for d, a in zip(digits, avails)) for t in targets)
+            for cols in col_sets:
+                cols = tuple(cols)
                 if cols in seen:
                     continue
                 seen.add(cols)
@@ -105,7 +112,7 @@ def time_plan_batch(Z, torch, k, digits, cols, xs, ys, stream, reps):
     return e0.elapsed_time(e1) / reps / len(jobs)
 
 
-def tune(ks, reps=5, out=None, verbose=True, max_candidates=None, kind="single"):
+def tune(ks, reps=5, out=None, verbose=True, max_candidates=None, kind="single", wide=False):
     import torch
 
     import zk_gpu_amd as Z
@@ -120,7 +127,7 @@ def tune(ks, reps=5, out=None, verbose=True, max_candidates=None, kind="single")
         ys = [torch.empty_like(x) for x in xs]
         for i, x in enumerate(xs):
             Z.fill_random(x.data_ptr(), n, 99 + i, stream=stream.cuda_stream)
-        cands = candidates(k, tile_log_max=TILE_LOG[kind])
+        cands = candidates(k, tile_log_max=TILE_LOG[kind], wide=wide)
         if max_candidates:
             cands = cands[:max_candidates]
         results = []
@@ -166,7 +173,7 @@ MIXES = {
 }
 
 
-def tune_batch_mix(mix="mixed64", reps=3, sweeps=2, out=None, verbose=True):
+def tune_batch_mix(mix="mixed64", reps=3, sweeps=2, out=None, verbose=True, wide=False):
     """Coordinate descent over the size classes of a mixed batch: for each
     class in turn, try every candidate split with the other classes fixed and
     keep the one that makes the whole batch fastest."""
@@ -209,7 +216,7 @@ def tune_batch_mix(mix="mixed64", reps=3, sweeps=2, out=None, verbose=True):
         print(f"start {best_ms:.4f} ms", flush=True)
     for sweep in range(sweeps):
         for k in classes:
-            for digits, cols in candidates(k, tile_log_max=TILE_LOG["batch"]):
+            for digits, cols in candidates(k, tile_log_max=TILE_LOG["batch"], wide=wide):
                 Z.plan_set(k, digits, cols, kind=Z.PLAN_BATCH)
                 ms = run()
                 if ms < best_ms:
@@ -260,11 +267,12 @@ if __name__ == "__main__":
     ap.add_argument("--max-candidates", type=int, default=None)
     ap.add_argument("--mix", default=None, help="tune batch plans on a mixed workload (mixed64, serve)")
     ap.add_argument("--sweeps", type=int, default=2)
+    ap.add_argument("--wide", action="store_true", help="let each pass choose its own tile width")
     a = ap.parse_args()
     sys.path.insert(0, os.path.dirname(HERE))
     t = time.time()
     if a.mix:
-        tune_batch_mix(a.mix, reps=a.reps, sweeps=a.sweeps, out=a.out)
+        tune_batch_mix(a.mix, reps=a.reps, sweeps=a.sweeps, out=a.out, wide=a.wide)
     else:
-        tune(_parse_ks(a.k), reps=a.reps, out=a.out, max_candidates=a.max_candidates, kind=a.kind)
+        tune(_parse_ks(a.k), reps=a.reps, out=a.out, max_candidates=a.max_candidates, kind=a.kind, wide=a.wide)
     print(f"autotune took {time.time() - t:.1f} s")
