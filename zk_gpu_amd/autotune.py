@@ -79,19 +79,45 @@ def candidates(k, min_tiles_log=8, max_passes=3, tile_log_max=12, wide=False):
     return out
 
 
-def time_plan(Z, torch, k, digits, cols, x, y, stream, reps):
-    Z.plan_set(k, digits, cols)
-    Z.lib().zkgpu_drop_tables(0)
-    Z.prepare_std(k, Z.OP_FWD)
-    Z.ntt_std_device(x.data_ptr(), y.data_ptr(), k, Z.OP_FWD, stream=stream.cuda_stream)
+# ops timed per repetition: forward, inverse and coset forward, the mix the
+# benchmark's step runs (their first and last passes differ)
+TUNE_OPS = (0, 1, 2)
+
+
+def time_plan(Z, torch, k, digits, cols, x, y, stream, reps, prepared=False):
+    """ms per transform (mean over the ops of TUNE_OPS)."""
+    if not prepared:
+        Z.plan_set(k, digits, cols)
+        Z.lib().zkgpu_drop_tables(0)
+        for op in TUNE_OPS:
+            Z.prepare_std(k, op)
+    for op in TUNE_OPS:
+        Z.ntt_std_device(x.data_ptr(), y.data_ptr(), k, op, stream=stream.cuda_stream)
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(stream)
     for _ in range(reps):
-        Z.ntt_std_device(x.data_ptr(), y.data_ptr(), k, Z.OP_FWD, stream=stream.cuda_stream)
+        for op in TUNE_OPS:
+            Z.ntt_std_device(x.data_ptr(), y.data_ptr(), k, op, stream=stream.cuda_stream)
     e1.record(stream)
     torch.cuda.synchronize()
-    return e0.elapsed_time(e1) / reps
+    return e0.elapsed_time(e1) / reps / len(TUNE_OPS)
+
+
+def confirm(Z, torch, k, finalists, x, y, stream, reps, runs=5):
+    """Median over `runs` interleaved timings of each finalist (single-transform
+    plans): the sweep's single timing of hundreds of candidates picks noise
+    between plans a few percent apart."""
+    samples = {i: [] for i in range(len(finalists))}
+    for _ in range(runs):
+        for i, (digits, cols) in enumerate(finalists):
+            Z.plan_set(k, digits, cols)
+            Z.lib().zkgpu_drop_tables(0)
+            for op in TUNE_OPS:
+                Z.prepare_std(k, op)
+            samples[i].append(time_plan(Z, torch, k, digits, cols, x, y, stream, reps, prepared=True))
+    med = [sorted(v)[len(v) // 2] for _, v in sorted(samples.items())]
+    return sorted((m, list(f[0]), list(f[1])) for m, f in zip(med, finalists))
 
 
 def time_plan_batch(Z, torch, k, digits, cols, xs, ys, stream, reps):
@@ -140,6 +166,17 @@ def tune(ks, reps=5, out=None, verbose=True, max_candidates=None, kind="single",
             if verbose:
                 print(f"k={k} digits={digits} cols={cols} {ms:.5f} ms", flush=True)
         results.sort()
+        if kind == "single":
+            # the five fastest and the plan in use now, re-timed
+            Z.plan_reset(k, kind=Z.PLAN_SINGLE)
+            npass, d0, c0 = Z.plan_get(k, Z.PLAN_SINGLE)
+            finalists = [(r[1], r[2]) for r in results[:5]]
+            if (d0, c0) not in finalists:
+                finalists.append((d0, c0))
+            results = confirm(Z, torch, k, finalists, xs[0], ys[0], stream, reps * 3)
+            if verbose:
+                for r in results:
+                    print(f"confirm k={k} digits={r[1]} cols={r[2]} {r[0]:.5f} ms", flush=True)
         best[k] = results[0]
         Z.plan_reset(k, kind=Z.PLAN_SINGLE if kind == "single" else Z.PLAN_BATCH)
         if verbose:
