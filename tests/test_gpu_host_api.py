@@ -70,3 +70,42 @@ def test_host_api_concurrent_callers(torch_cuda):
     assert not errs, errs
     for g_, w_ in zip(got, want):
         assert (g_ == w_).all()
+
+
+@pytest.mark.parametrize("log_n", [8, 12, 16])
+def test_reference_caller_protocol(torch_cuda, log_n):
+    """The reference's own caller (src/main.rs:50-91, 128-162): elements
+    Fr::from_raw([i, 9720557559516996230, 14346186778711300467,
+    2065298866258849470]) in Montgomery form, omega derived through the
+    extended domain (j = 3 -> extended_k = k + 1, squared back down), the
+    sparse twiddle table generate_twiddle_lookup_table(omega, k, 10, true)
+    passed along, parallel_fft_api in place. The result equals the oracle and,
+    at sampled indices, the definition X_j = a(omega^j)."""
+    n = 1 << log_n
+    hi = 9720557559516996230 << 64 | 14346186778711300467 << 128 | 2065298866258849470 << 192
+    vals = [i | hi for i in range(n)]
+    data = np.concatenate([O.elem(v) for v in vals])
+    # omega: the 2^28-adic root squared down to the extended domain, then to 2^k
+    ext_k = log_n
+    while (1 << ext_k) < n * 2:
+        ext_k += 1
+    w_ext = O.value(O.root_of_unity(28)[0])
+    for _ in range(ext_k, 28):
+        w_ext = w_ext * w_ext % R
+    w = w_ext
+    for _ in range(log_n, ext_k):
+        w = w * w % R
+    omega = O.elem(w)
+    assert (omega == O.root_of_unity(log_n)).all()
+    twiddle = O.twiddle_lut(omega, log_n)
+    a = data.copy()
+    rc = Z.lib().parallel_fft_api(0, Z._ptr(a), Z._ptr(omega.reshape(4)), Z._ptr(twiddle), n, log_n,
+                                  twiddle.shape[0])
+    assert rc == 0
+    assert (a == O.fft(data, log_n, threads=4)).all()
+    for j in (0, 1, n // 2 + 3, n - 1):
+        x = pow(w, j, R)
+        acc = 0
+        for v in reversed(vals):
+            acc = (acc * x + v) % R
+        assert O.value(a[j]) == acc, j
