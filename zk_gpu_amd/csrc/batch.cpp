@@ -33,7 +33,8 @@ struct Batch {
   void* meta_ptr = nullptr;
   std::vector<char> host_meta;    // source of the (asynchronous) meta upload
   struct Level {
-    uint32_t desc_off, ndesc, total, grid;
+    uint32_t desc_off = 0, ndesc = 0, total = 0, grid = 0;  // total: queue items
+    uint32_t tiles = 0;
   };
   std::vector<Level> levels;
   uint32_t* d_counters = nullptr;
@@ -132,7 +133,8 @@ int batch_build(int device, const zkgpu_job* cjobs, uint32_t count, Batch** out,
       if (lvl < per_job[i].size()) {
         starts.push_back(total);
         all.push_back(per_job[i][lvl]);
-        total += per_job[i][lvl].num_tiles;
+        total += batch_items(per_job[i][lvl]);
+        L.tiles += per_job[i][lvl].num_tiles;
       }
     }
     L.ndesc = (uint32_t)all.size() - L.desc_off;
@@ -245,7 +247,7 @@ int zkgpu_batch_info(uint64_t handle, uint32_t* levels, uint32_t* tiles, uint64_
   if (levels) *levels = (uint32_t)b->levels.size();
   if (tiles) {
     uint32_t t = 0;
-    for (const auto& L : b->levels) t += L.total;
+    for (const auto& L : b->levels) t += L.tiles;
     *tiles = t;
   }
   if (elements) *elements = b->elements;

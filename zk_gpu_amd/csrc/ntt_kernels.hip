@@ -376,15 +376,17 @@ __device__ __forceinline__ void ntt_rounds(fe (&u)[TileShape<LOGN>::E], const Pa
   }
 }
 
+// tid: the thread's index within the tile; valid: the tile exists (the
+// last group of a batch descriptor may run past its tiles). Inactive threads
+// still take every barrier.
 template <int LOGN>
-__device__ __forceinline__ void ntt_tile(const PassDesc& d, uint32_t tile, uint32_t* lds) {
+__device__ __forceinline__ void ntt_tile(const PassDesc& d, uint32_t tile, uint32_t* lds, uint32_t tid, bool valid) {
   using Sh = TileShape<LOGN>;
   TileCtx t;
   t.logN = LOGN;
   t.logC = d.log_c;
   const uint32_t nthreads = 1u << (LOGN + t.logC - Sh::LE);
-  const uint32_t tid = threadIdx.x;
-  t.active = tid < nthreads;
+  t.active = valid && tid < nthreads;
   t.c = tid & ((1u << t.logC) - 1u);
   t.i = tid >> t.logC;
   t.plane = 1u << (LOGN + t.logC);
@@ -407,30 +409,42 @@ constexpr int MAX_TILE_THREADS = 1 << (TILE_LOG_MAX - 3);
 template <int LOGN>
 __global__ void __launch_bounds__(MAX_TILE_THREADS) ntt_pass_kernel(PassDesc d) {
   extern __shared__ uint32_t lds[];
-  ntt_tile<LOGN>(d, xcd_tile(blockIdx.x, d.num_tiles), lds);
+  ntt_tile<LOGN>(d, xcd_tile(blockIdx.x, d.num_tiles), lds, threadIdx.x, true);
 }
 
 // Persistent pass kernel for batches: a grid of resident workgroups drains a
-// queue of tiles (any job, any tile length up to 2^BATCH_TILE_LOG elements)
-// through an atomic counter. Items are ordered by descriptor; tile_start is
-// the exclusive prefix sum of the descriptors' tile counts. Every workgroup
-// leaves the loop once the counter passes `total`.
+// queue of items (any job, any tile length up to 2^BATCH_TILE_LOG elements)
+// through an atomic counter. An item is a group of consecutive tiles of one
+// descriptor that together fill the workgroup's BATCH_THREADS threads (a
+// 2^10-element tile takes 128 threads, so an item holds two of them); each
+// tile of the group has its own slice of the LDS planes. Items are ordered by
+// descriptor; tile_start is the exclusive prefix sum of the descriptors' item
+// counts (batch_items). Every workgroup leaves the loop once the counter
+// passes `total`.
 //
-// LDS layout: the tile's three limb planes from byte 0 (16-byte aligned, as
+// LDS layout: the tiles' three limb planes from byte 0 (16-byte aligned, as
 // in the single-pass kernel: the planes are accessed with 16-byte DS
 // operations), then one word holding the claimed item.
 constexpr uint32_t BATCH_TILE_LOG = 11;  // 2048-element tiles: 72 KiB of LDS, two workgroups per CU
 constexpr int BATCH_THREADS = 1 << (BATCH_TILE_LOG - 3);
 constexpr uint32_t BATCH_ITEM_WORD = NLIMBS << BATCH_TILE_LOG;  // LDS word index of the item slot
 
+constexpr uint32_t BATCH_THREADS_LOG = BATCH_TILE_LOG - 3;
+
+// Item `group` of descriptor d: tiles group * G .. group * G + G - 1, G =
+// BATCH_THREADS / (threads per tile).
 template <int L>
-__device__ __forceinline__ void tile_dispatch(const PassDesc& d, uint32_t tile, uint32_t* lds) {
+__device__ __forceinline__ void tile_dispatch(const PassDesc& d, uint32_t group, uint32_t* lds) {
   if constexpr (L <= (int)BATCH_TILE_LOG) {
     if (d.log_n_tile == (uint32_t)L) {
-      ntt_tile<L>(d, tile, lds);
+      const uint32_t lgt = L + d.log_c - TileShape<L>::LE;  // log2 threads per tile (<= BATCH_THREADS_LOG)
+      const uint32_t sub = threadIdx.x >> lgt;
+      const uint32_t tile = (group << (BATCH_THREADS_LOG - lgt)) + sub;
+      ntt_tile<L>(d, tile, lds + sub * (NLIMBS << (L + d.log_c)), threadIdx.x & ((1u << lgt) - 1u),
+                  tile < d.num_tiles);
       return;
     }
-    tile_dispatch<L + 1>(d, tile, lds);
+    tile_dispatch<L + 1>(d, group, lds);
   }
 }
 
@@ -590,6 +604,11 @@ uint32_t tile_threads(uint32_t log_n_tile, uint32_t log_c) {
 }
 uint32_t tile_lds_bytes(uint32_t log_n_tile, uint32_t log_c) {
   return (uint32_t)(4u * NLIMBS) << (log_n_tile + log_c);
+}
+// queue items of a batch descriptor: groups of tiles that fill BATCH_THREADS
+uint32_t batch_items(const PassDesc& d) {
+  const uint32_t per_item = (uint32_t)BATCH_THREADS / tile_threads(d.log_n_tile, d.log_c);
+  return (d.num_tiles + per_item - 1) / per_item;
 }
 // planes of the largest batch tile, then the 16-byte item slot
 static uint32_t persistent_lds_bytes() { return 4u * BATCH_ITEM_WORD + 16u; }

@@ -324,7 +324,7 @@ class Server {
       const PassDesc& d = j->descs[j->next];
       hd[nd] = d;
       hts[nd] = total;
-      total += d.num_tiles;
+      total += batch_items(d);
       ++nd;
       if (j->next == 0) j->t_start = t;
       if (++j->next == j->npass) it.finishing.push_back(j);
