@@ -139,8 +139,8 @@ int batch_build(int device, const zkgpu_job* cjobs, uint32_t count, Batch** out,
     }
     L.ndesc = (uint32_t)all.size() - L.desc_off;
     L.total = total;
-    // resident workgroups: two 2048-element tiles (72 KiB of LDS each) per CU
-    L.grid = std::min<uint32_t>(total, (uint32_t)cus * 2);
+    // one resident workgroup per LDS/register slot of every CU
+    L.grid = std::min<uint32_t>(total, (uint32_t)cus * persistent_wgs_per_cu());
     b->levels.push_back(L);
   }
   // one device buffer: descriptors | tile prefix sums | counters

@@ -51,7 +51,7 @@ struct PassDesc {
   uint32_t out_lo, out_hi, out_ts, out_cs;
 };
 
-// Largest tile (elements): N * C <= 2^TILE_LOG_MAX; 4096 x 36 B = 144 KiB of LDS.
+// Largest tile (elements): N * C <= 2^TILE_LOG_MAX; 4096 x 20 B = 80 KiB of LDS.
 constexpr uint32_t TILE_LOG_MAX = 12;
 
 }  // namespace zkgpu

@@ -4,12 +4,12 @@
 // ntt_desc.hpp). Each thread keeps 8 elements in 29-bit-limb form
 // (8 x 9 VGPRs) and the tile runs Stockham (self-sorting) rounds of radix
 // 8 (the last round takes the remaining radix 2/4): round 1 reads HBM,
-// middle rounds exchange through LDS (9 dword planes, conflict-free for
-// consecutive positions), the last round writes HBM. Every pass therefore
-// reads and writes each element once. Inside a round a thread multiplies its
-// inputs by the round twiddles omega_{pR}^{jk}, runs a radix-2
-// decimation-in-frequency network over its R registers and scatters the
-// (bit-reversal relabelled) outputs.
+// middle rounds exchange through LDS (dword planes, conflict-free for
+// consecutive positions, limbs in two phases), the last round writes HBM.
+// Every pass therefore reads and writes each element once. Inside a round a
+// thread multiplies its inputs by the round twiddles omega_{pR}^{jk}, runs a
+// radix-2 decimation-in-frequency network over its R registers and scatters
+// the (bit-reversal relabelled) outputs.
 //
 // Pass-level fusions: the coset pre-scale (a[i] *= c^i, reference
 // cuda/fft.cu:223-232) on the first load; the inter-pass twiddle, the
@@ -75,6 +75,50 @@ __device__ __forceinline__ void store_planar(void* base, uint32_t plane, uint32_
   a[idx] = make_uint4(v.l[0], v.l[1], v.l[2], v.l[3]);
   b[idx] = make_uint4(v.l[4], v.l[5], v.l[6], v.l[7]);
   c[idx] = v.l[8];
+}
+
+// How the rounds of a tile exchange data through LDS.
+//   two_phase: limbs 0-3 (a uint4 plane) and limb 4 (a u32 plane), then
+//              limbs 5-8 in the same uint4 plane, each phase closed by a
+//              barrier: 20 B per element, so three 2048-element tiles share a
+//              CU, for two more barriers per exchange. Otherwise all nine
+//              limbs at once in the planes of store_planar (36 B per element).
+//   prefetch:  the round twiddles are loaded before the LDS reads and the
+//              barrier (more registers live across it).
+template <bool TWO_PHASE, bool PREFETCH>
+struct Xchg {
+  static constexpr bool two_phase = TWO_PHASE;
+  static constexpr bool prefetch = PREFETCH;
+  static constexpr uint32_t words = TWO_PHASE ? 5u : (uint32_t)NLIMBS;  // LDS words per tile element
+};
+
+// PH 0: limbs 0-4, PH 1: limbs 5-8 (the two phases), PH 2: all nine limbs
+template <int PH>
+__device__ __forceinline__ void lds_put(uint32_t* lds, uint32_t plane, uint32_t idx, const fe& v) {
+  uint4* a = (uint4*)lds;
+  if constexpr (PH == 2) {
+    store_planar(lds, plane, idx, v);
+  } else if constexpr (PH == 0) {
+    a[idx] = make_uint4(v.l[0], v.l[1], v.l[2], v.l[3]);
+    ((uint32_t*)(a + plane))[idx] = v.l[4];
+  } else {
+    a[idx] = make_uint4(v.l[5], v.l[6], v.l[7], v.l[8]);
+  }
+}
+template <int PH>
+__device__ __forceinline__ void lds_get(const uint32_t* lds, uint32_t plane, uint32_t idx, fe& v) {
+  const uint4* a = (const uint4*)lds;
+  if constexpr (PH == 2) {
+    v = load_planar(lds, plane, idx);
+    return;
+  }
+  const uint4 x = a[idx];
+  if constexpr (PH == 0) {
+    v.l[0] = x.x; v.l[1] = x.y; v.l[2] = x.z; v.l[3] = x.w;
+    v.l[4] = ((const uint32_t*)(a + plane))[idx];
+  } else {
+    v.l[5] = x.x; v.l[6] = x.y; v.l[7] = x.z; v.l[8] = x.w;
+  }
 }
 
 template <int R>
@@ -243,9 +287,50 @@ struct TileCtx {
   bool active;
 };
 
-// PREFETCH: load the round twiddles before the LDS reads and the barrier
-// (more registers live across the barrier; used where that fits in 256 VGPRs).
-template <int E, int R, bool PREFETCH>
+// Two-phase exchange helpers. A round of radix R = 2^LR at stride 2^logp
+// writes its register s*R + bitrev(m) to tile position base(s) + m*2^logp;
+// the next round reads its register s*R + j from position iv(s) + j*N/R.
+template <int E, int R, int PH>
+__device__ __forceinline__ void lds_write_round(const fe (&u)[E], const TileCtx& t, uint32_t logp, uint32_t* lds) {
+  constexpr int S = E / R;
+  constexpr int LR = Log2<R>::value;
+  constexpr int LE = Log2<E>::value;
+  const uint32_t NE = 1u << (t.logN - LE);
+  const uint32_t pmask = (1u << logp) - 1u;
+  static_for<S>([&](auto sc) {
+    constexpr int s = decltype(sc)::value;
+    const uint32_t iv = t.i + s * NE;
+    const uint32_t k = iv & pmask;
+    const uint32_t base = ((iv - k) << LR) + k;
+    static_for<R>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      const uint32_t pos = base + ((uint32_t)m << logp);
+      lds_put<PH>(lds, t.plane, t.c + (pos << t.logC), u[s * R + bitrev_c(m, LR)]);
+    });
+  });
+}
+template <int E, int R, int PH>
+__device__ __forceinline__ void lds_read_round(fe (&u)[E], const TileCtx& t, const uint32_t* lds) {
+  constexpr int S = E / R;
+  constexpr int LR = Log2<R>::value;
+  constexpr int LE = Log2<E>::value;
+  const uint32_t NE = 1u << (t.logN - LE);
+  const uint32_t NR = 1u << (t.logN - LR);
+  static_for<S>([&](auto sc) {
+    constexpr int s = decltype(sc)::value;
+    const uint32_t iv = t.i + s * NE;
+    static_for<R>([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
+      lds_get<PH>(lds, t.plane, t.c + ((iv + j * NR) << t.logC), u[s * R + j]);
+    });
+  });
+}
+
+// One Stockham round of radix R on the thread's E registers. A round after
+// the first takes its inputs from LDS (exchange X) and multiplies them by the
+// round twiddles; the last round writes HBM, the others write their outputs
+// (limbs 0-4 of them, with a two-phase exchange) to LDS.
+template <int E, int R, class X>
 __device__ __forceinline__ void ntt_round(fe (&u)[E], const PassDesc& d, const TileCtx& t, uint32_t logp,
                                           bool first, bool last, uint32_t* lds) {
   constexpr int S = E / R;
@@ -273,40 +358,40 @@ __device__ __forceinline__ void ntt_round(fe (&u)[E], const PassDesc& d, const T
       });
     }
   } else {
-    // round twiddles omega_{pR}^{jk}: issued ahead of the LDS reads and the
-    // barrier so that their latency overlaps them
-    fe tw[S][R];
+    // round twiddles omega_{pR}^{jk}
     const uint32_t sh = d.root_log - (logp + LR);
-    auto load_tw = [&] {
-      static_for<S>([&](auto sc) {
-        constexpr int s = decltype(sc)::value;
-        const uint32_t k = (t.i + s * NE) & pmask;
-        static_for<R, 1>([&](auto jc) {
-          constexpr int j = decltype(jc)::value;
-          tw[s][j] = load_limbs48(roots, (uint64_t)(j * k) << sh);
-        });
-      });
+    auto twiddle = [&](int s, int j) {
+      return load_limbs48(roots, (uint64_t)(j * ((t.i + s * NE) & pmask)) << sh);
     };
+    fe tw[X::prefetch ? S : 1][X::prefetch ? R : 1];
     if (t.active) {
-      if constexpr (PREFETCH) load_tw();
-      static_for<S>([&](auto sc) {
-        constexpr int s = decltype(sc)::value;
-        const uint32_t iv = t.i + s * NE;
-        static_for<R>([&](auto jc) {
-          constexpr int j = decltype(jc)::value;
-          u[s * R + j] = load_planar(lds, t.plane, t.c + ((iv + j * NR) << t.logC));
+      if constexpr (X::prefetch) {
+        static_for<S>([&](auto sc) {
+          static_for<R, 1>([&](auto jc) {
+            tw[decltype(sc)::value][decltype(jc)::value] = twiddle(decltype(sc)::value, decltype(jc)::value);
+          });
         });
-      });
+      }
+      lds_read_round<E, R, X::two_phase ? 0 : 2>(u, t, lds);
     }
-    __syncthreads();  // every LDS read of this round precedes any overwrite
+    __syncthreads();  // every LDS read precedes the next overwrite
+    if constexpr (X::two_phase) {
+      // limbs 5-8: the previous round (radix E, since only a tile's last
+      // round is shorter) writes them where it wrote limbs 0-4, this round
+      // reads them into the registers that now hold its limbs 0-4
+      if (t.active) lds_write_round<E, E, 1>(u, t, logp - LE, lds);
+      __syncthreads();
+      if (t.active) lds_read_round<E, R, 1>(u, t, lds);
+      __syncthreads();  // every LDS read of this round precedes any overwrite
+    }
     if (t.active) {
-      if constexpr (!PREFETCH) load_tw();
       static_for<S>([&](auto sc) {
         constexpr int s = decltype(sc)::value;
         reduce_value(u[s * R]);
         static_for<R, 1>([&](auto jc) {
           constexpr int j = decltype(jc)::value;
-          u[s * R + j] = mont_mul(u[s * R + j], tw[s][j]);
+          if constexpr (X::prefetch) u[s * R + j] = mont_mul(u[s * R + j], tw[s][j]);
+          else u[s * R + j] = mont_mul(u[s * R + j], twiddle(s, j));
         });
       });
     }
@@ -346,7 +431,7 @@ __device__ __forceinline__ void ntt_round(fe (&u)[E], const PassDesc& d, const T
             store_words(d.out, off, w);
           }
         } else {
-          store_planar(lds, t.plane, t.c + (pos << t.logC), v);
+          lds_put<X::two_phase ? 0 : 2>(lds, t.plane, t.c + (pos << t.logC), v);
         }
       });
     });
@@ -361,25 +446,24 @@ template <int LOGN> struct TileShape {
   static constexpr int FULL = LOGN / LE;                 // rounds of radix E
   static constexpr int REM = LOGN - FULL * LE;           // log2 radix of the last round (0: none)
   static constexpr int ROUNDS = FULL + (REM ? 1 : 0);
-  static constexpr bool PREFETCH = LOGN <= 9;  // larger tiles would spill at 2 waves/SIMD
 };
 
-template <int LOGN, int RIDX>
+template <int LOGN, int RIDX, class X>
 __device__ __forceinline__ void ntt_rounds(fe (&u)[TileShape<LOGN>::E], const PassDesc& d, const TileCtx& t,
                                            uint32_t* lds) {
   using Sh = TileShape<LOGN>;
   if constexpr (RIDX < Sh::ROUNDS) {
     constexpr int LR = RIDX < Sh::FULL ? Sh::LE : Sh::REM;
     constexpr uint32_t LOGP = (uint32_t)(RIDX * Sh::LE);
-    ntt_round<Sh::E, (1 << LR), Sh::PREFETCH>(u, d, t, LOGP, RIDX == 0, RIDX == Sh::ROUNDS - 1, lds);
-    ntt_rounds<LOGN, RIDX + 1>(u, d, t, lds);
+    ntt_round<Sh::E, (1 << LR), X>(u, d, t, LOGP, RIDX == 0, RIDX == Sh::ROUNDS - 1, lds);
+    ntt_rounds<LOGN, RIDX + 1, X>(u, d, t, lds);
   }
 }
 
 // tid: the thread's index within the tile; valid: the tile exists (the
 // last group of a batch descriptor may run past its tiles). Inactive threads
 // still take every barrier.
-template <int LOGN>
+template <int LOGN, class X>
 __device__ __forceinline__ void ntt_tile(const PassDesc& d, uint32_t tile, uint32_t* lds, uint32_t tid, bool valid) {
   using Sh = TileShape<LOGN>;
   TileCtx t;
@@ -394,7 +478,7 @@ __device__ __forceinline__ void ntt_tile(const PassDesc& d, uint32_t tile, uint3
   t.in_base = tlo * d.in_lo + thi * d.in_hi + t.c * d.in_cs;
   t.out_base = tlo * d.out_lo + thi * d.out_hi + t.c * d.out_cs;
   fe u[Sh::E];
-  ntt_rounds<LOGN, 0>(u, d, t, lds);
+  ntt_rounds<LOGN, 0, X>(u, d, t, lds);
 }
 
 // Map workgroups so that consecutive tiles land on the same XCD: workgroups
@@ -406,10 +490,14 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n) {
 
 constexpr int MAX_TILE_THREADS = 1 << (TILE_LOG_MAX - 3);
 
+// Single-pass tiles: two-phase exchange and at most 168 VGPRs (three waves
+// per SIMD, so no twiddle prefetch): three 2048-element tiles share a CU.
+using PassXchg = Xchg<true, false>;
+
 template <int LOGN>
-__global__ void __launch_bounds__(MAX_TILE_THREADS) ntt_pass_kernel(PassDesc d) {
+__global__ void __launch_bounds__(MAX_TILE_THREADS, 3) ntt_pass_kernel(PassDesc d) {
   extern __shared__ uint32_t lds[];
-  ntt_tile<LOGN>(d, xcd_tile(blockIdx.x, d.num_tiles), lds, threadIdx.x, true);
+  ntt_tile<LOGN, PassXchg>(d, xcd_tile(blockIdx.x, d.num_tiles), lds, threadIdx.x, true);
 }
 
 // Persistent pass kernel for batches: a grid of resident workgroups drains a
@@ -425,11 +513,18 @@ __global__ void __launch_bounds__(MAX_TILE_THREADS) ntt_pass_kernel(PassDesc d) 
 // LDS layout: the tiles' three limb planes from byte 0 (16-byte aligned, as
 // in the single-pass kernel: the planes are accessed with 16-byte DS
 // operations), then one word holding the claimed item.
-constexpr uint32_t BATCH_TILE_LOG = 11;  // 2048-element tiles: 72 KiB of LDS, two workgroups per CU
+// The persistent kernel inlines every tile length and runs two workgroups
+// per CU (three would spill), so it exchanges all limbs at once and prefetches
+// the twiddles of tiles up to 2^9 elements.
+template <int L>
+using BatchXchg = Xchg<false, (L <= 9)>;
+constexpr uint32_t BATCH_TILE_LOG = 11;  // 2048-element tiles: 72 KiB of LDS
 constexpr int BATCH_THREADS = 1 << (BATCH_TILE_LOG - 3);
-constexpr uint32_t BATCH_ITEM_WORD = NLIMBS << BATCH_TILE_LOG;  // LDS word index of the item slot
+constexpr uint32_t BATCH_ITEM_WORD = (uint32_t)NLIMBS << BATCH_TILE_LOG;  // LDS word index of the item slot
 
 constexpr uint32_t BATCH_THREADS_LOG = BATCH_TILE_LOG - 3;
+// resident workgroups per CU (one wave per SIMD each)
+constexpr int PERSIST_WGS = 2;
 
 // Item `group` of descriptor d: tiles group * G .. group * G + G - 1, G =
 // BATCH_THREADS / (threads per tile).
@@ -440,7 +535,7 @@ __device__ __forceinline__ void tile_dispatch(const PassDesc& d, uint32_t group,
       const uint32_t lgt = L + d.log_c - TileShape<L>::LE;  // log2 threads per tile (<= BATCH_THREADS_LOG)
       const uint32_t sub = threadIdx.x >> lgt;
       const uint32_t tile = (group << (BATCH_THREADS_LOG - lgt)) + sub;
-      ntt_tile<L>(d, tile, lds + sub * (NLIMBS << (L + d.log_c)), threadIdx.x & ((1u << lgt) - 1u),
+      ntt_tile<L, BatchXchg<L>>(d, tile, lds + sub * (BatchXchg<L>::words << (L + d.log_c)), threadIdx.x & ((1u << lgt) - 1u),
                   tile < d.num_tiles);
       return;
     }
@@ -448,7 +543,7 @@ __device__ __forceinline__ void tile_dispatch(const PassDesc& d, uint32_t group,
   }
 }
 
-__global__ void __launch_bounds__(BATCH_THREADS, 2) ntt_persistent_kernel(const PassDesc* __restrict__ descs,
+__global__ void __launch_bounds__(BATCH_THREADS, PERSIST_WGS) ntt_persistent_kernel(const PassDesc* __restrict__ descs,
                                                                        const uint32_t* __restrict__ tile_start,
                                                                        uint32_t ndesc, uint32_t total,
                                                                        uint32_t* counter) {
@@ -603,13 +698,14 @@ uint32_t tile_threads(uint32_t log_n_tile, uint32_t log_c) {
   return 1u << (log_n_tile + log_c - le);
 }
 uint32_t tile_lds_bytes(uint32_t log_n_tile, uint32_t log_c) {
-  return (uint32_t)(4u * NLIMBS) << (log_n_tile + log_c);
+  return (uint32_t)(4u * PassXchg::words) << (log_n_tile + log_c);
 }
 // queue items of a batch descriptor: groups of tiles that fill BATCH_THREADS
 uint32_t batch_items(const PassDesc& d) {
   const uint32_t per_item = (uint32_t)BATCH_THREADS / tile_threads(d.log_n_tile, d.log_c);
   return (d.num_tiles + per_item - 1) / per_item;
 }
+uint32_t persistent_wgs_per_cu() { return (uint32_t)PERSIST_WGS; }
 // planes of the largest batch tile, then the 16-byte item slot
 static uint32_t persistent_lds_bytes() { return 4u * BATCH_ITEM_WORD + 16u; }
 
@@ -704,7 +800,7 @@ hipError_t launch_mad_probe(uint64_t* out, uint32_t blocks, uint32_t seed, hipSt
 }
 
 hipError_t configure_kernels() {
-  const int max_lds = (int)(4u * NLIMBS) << TILE_LOG_MAX;
+  const int max_lds = (int)(4u * PassXchg::words) << TILE_LOG_MAX;
   hipError_t e = hipSuccess;
 #define X(L)                                                                                              \
   if (e == hipSuccess)                                                                                    \

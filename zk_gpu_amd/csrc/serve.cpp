@@ -141,7 +141,7 @@ class Server {
     if (hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking) != hipSuccess) return ZK_ERR_HIP;
     int cus = 256;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_);
-    grid_cap_ = (uint32_t)cus * 2;  // two resident 2048-element tiles per CU
+    grid_cap_ = (uint32_t)cus * persistent_wgs_per_cu();  // resident workgroups of the persistent kernel
     // staging layout of an iteration: counter | tile starts | descriptors;
     // only the used part of it is copied
     ts_off_ = 256;
