@@ -62,8 +62,24 @@ ZK_HD uint32_t p32(int i) {
        : i == 5 ? P32[5] : i == 6 ? P32[6] : P32[7];
 }
 
+// Keeps a column accumulator opaque to the compiler's reassociation, so that
+// a column is one multiply-add chain that starts from the carry.
+template <bool CHAIN>
+ZK_HD void chain_point(uint64_t& acc) {
+#ifdef __HIP_DEVICE_COMPILE__
+  if constexpr (CHAIN) asm("" : "+v"(acc));
+#endif
+  (void)acc;
+}
+
 // Montgomery product a * b * 2^-261 mod r (not reduced), product scanning
 // with the reduction interleaved per column (FIPS, Koc-Acar-Kaliski 1996).
+// CHAIN = false: the compiler splits each column into chains that start from
+// zero and adds the carry last (more independent work per wave, one 64-bit
+// add more per column). CHAIN = true: one chain per column from the carry
+// (fewer instructions and registers, longer dependences; it pays where four
+// waves per SIMD hide them).
+template <bool CHAIN = false>
 ZK_HD fe mont_mul(const fe& a, const fe& b) {
   uint32_t m[NLIMBS];
   fe o;
@@ -75,12 +91,14 @@ ZK_HD fe mont_mul(const fe& a, const fe& b) {
       const int j = k - i;
       if (j < 0 || j >= NLIMBS) continue;
       acc += (uint64_t)a.l[i] * b.l[j];
+      chain_point<CHAIN>(acc);
     }
 #pragma unroll
     for (int i = 0; i < NLIMBS; ++i) {
       const int j = k - i;
       if (i >= k || j < 1 || j >= NLIMBS) continue;
       acc += (uint64_t)m[i] * p29(j);
+      chain_point<CHAIN>(acc);
     }
     if (k < NLIMBS) {
       m[k] = ((uint32_t)acc * NINV29) & LIMB_MASK;

@@ -85,10 +85,12 @@ __device__ __forceinline__ void store_planar(void* base, uint32_t plane, uint32_
 //              limbs at once in the planes of store_planar (36 B per element).
 //   prefetch:  the round twiddles are loaded before the LDS reads and the
 //              barrier (more registers live across it).
-template <bool TWO_PHASE, bool PREFETCH>
+//   chain:     the multiplier's single-chain column form (mont_mul<true>).
+template <bool TWO_PHASE, bool PREFETCH, bool CHAIN>
 struct Xchg {
   static constexpr bool two_phase = TWO_PHASE;
   static constexpr bool prefetch = PREFETCH;
+  static constexpr bool chain = CHAIN;
   static constexpr uint32_t words = TWO_PHASE ? 5u : (uint32_t)NLIMBS;  // LDS words per tile element
 };
 
@@ -237,7 +239,7 @@ __device__ __forceinline__ void fix_bound(fe& a) {
 }
 
 // the sub<C> constants are instantiated per butterfly; C is at most MUL_IN_VMAX + 1
-template <int S, int R, int H>
+template <int S, int R, int H, bool CHAIN>
 __device__ __forceinline__ void dif_stage(fe* u, const uint4* roots, uint32_t root_log) {
   constexpr int LR = Log2<R>::value;
   constexpr int BASE = S * (R / 2);
@@ -257,7 +259,7 @@ __device__ __forceinline__ void dif_stage(fe* u, const uint4* roots, uint32_t ro
       const fe x = u[b + j], y = u[b + j + H];
       u[b + j] = add(x, y);
       if constexpr (j == 0) u[b + j + H] = sub<C>(x, y);
-      else u[b + j + H] = mont_mul(sub<C>(x, y), w);
+      else u[b + j + H] = mont_mul<CHAIN>(sub<C>(x, y), w);
     });
   });
 }
@@ -265,12 +267,12 @@ __device__ __forceinline__ void dif_stage(fe* u, const uint4* roots, uint32_t ro
 // Length-R DFT in registers: natural order in, bit-reversed order out.
 // omega_R^e = roots[e << (root_log - log R)]. Inputs normalized with V < 6r;
 // outputs satisfy the multiplier's operand contract.
-template <int R>
+template <int R, bool CHAIN>
 __device__ __forceinline__ void dft_regs(fe* u, const uint4* roots, uint32_t root_log) {
   constexpr int LR = Log2<R>::value;
   static_for<LR>([&](auto sc) {
     constexpr int s = decltype(sc)::value;
-    dif_stage<s, R, (R >> (s + 1))>(u, roots, root_log);
+    dif_stage<s, R, (R >> (s + 1)), CHAIN>(u, roots, root_log);
   });
   static_for<R>([&](auto ic) {
     constexpr int i = decltype(ic)::value;
@@ -351,7 +353,7 @@ __device__ __forceinline__ void ntt_round(fe (&u)[E], const PassDesc& d, const T
           constexpr int j = decltype(jc)::value;
           const uint32_t off = t.in_base + (iv + j * NR) * d.in_ts;
           fe v = in_planar ? load_planar(d.in, d.in_plane, off) : load_packed(d.in, off);
-          if (d.pre) v = mont_mul(v, load_packed(d.pre, off & d.pre_mask));
+          if (d.pre) v = mont_mul<X::chain>(v, load_packed(d.pre, off & d.pre_mask));
           else if (in_planar) reduce_value(v);
           u[s * R + j] = v;
         });
@@ -390,14 +392,14 @@ __device__ __forceinline__ void ntt_round(fe (&u)[E], const PassDesc& d, const T
         reduce_value(u[s * R]);
         static_for<R, 1>([&](auto jc) {
           constexpr int j = decltype(jc)::value;
-          if constexpr (X::prefetch) u[s * R + j] = mont_mul(u[s * R + j], tw[s][j]);
-          else u[s * R + j] = mont_mul(u[s * R + j], twiddle(s, j));
+          if constexpr (X::prefetch) u[s * R + j] = mont_mul<X::chain>(u[s * R + j], tw[s][j]);
+          else u[s * R + j] = mont_mul<X::chain>(u[s * R + j], twiddle(s, j));
         });
       });
     }
   }
   if (t.active) {
-    static_for<S>([&](auto sc) { dft_regs<R>(&u[decltype(sc)::value * R], roots, d.root_log); });
+    static_for<S>([&](auto sc) { dft_regs<R, X::chain>(&u[decltype(sc)::value * R], roots, d.root_log); });
     static_for<S>([&](auto sc) {
       constexpr int s = decltype(sc)::value;
       const uint32_t iv = t.i + s * NE;
@@ -414,12 +416,12 @@ __device__ __forceinline__ void ntt_round(fe (&u)[E], const PassDesc& d, const T
             // pass multiplies them by its pre-twiddle on load
             store_planar(d.out, d.out_plane, off, v);
           } else {
-            if (d.post) v = mont_mul(v, load_packed(d.post, off));
+            if (d.post) v = mont_mul<X::chain>(v, load_packed(d.post, off));
             if (d.flags & PASS_USCALE) {
               fe sc;
 #pragma unroll
               for (int q = 0; q < NLIMBS; ++q) sc.l[q] = d.scale[q];
-              v = mont_mul(v, sc);
+              v = mont_mul<X::chain>(v, sc);
             }
             uint32_t w[8];
             if (d.flags & PASS_CANON) {
@@ -490,14 +492,25 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n) {
 
 constexpr int MAX_TILE_THREADS = 1 << (TILE_LOG_MAX - 3);
 
-// Single-pass tiles: two-phase exchange and at most 168 VGPRs (three waves
-// per SIMD, so no twiddle prefetch): three 2048-element tiles share a CU.
-using PassXchg = Xchg<true, false>;
+// Single-pass tiles, two-phase exchange, no twiddle prefetch, in two
+// families chosen by tile size at launch:
+//   tiles up to 2^11 elements (<= 256 threads): at most 128 VGPRs, four waves
+//     per SIMD (four 40 KiB tiles fill a CU's 160 KiB), single-chain columns;
+//   2^12-element tiles (512 threads): at most 168 VGPRs, three waves per
+//     SIMD, split columns (one such tile per CU).
+constexpr uint32_t NARROW_TILE_LOG = 11;
+using PassXchg = Xchg<true, false, false>;
+using NarrowXchg = Xchg<true, false, true>;
 
 template <int LOGN>
 __global__ void __launch_bounds__(MAX_TILE_THREADS, 3) ntt_pass_kernel(PassDesc d) {
   extern __shared__ uint32_t lds[];
   ntt_tile<LOGN, PassXchg>(d, xcd_tile(blockIdx.x, d.num_tiles), lds, threadIdx.x, true);
+}
+template <int LOGN>
+__global__ void __launch_bounds__(1 << (NARROW_TILE_LOG - 3), 4) ntt_pass_kernel_narrow(PassDesc d) {
+  extern __shared__ uint32_t lds[];
+  ntt_tile<LOGN, NarrowXchg>(d, xcd_tile(blockIdx.x, d.num_tiles), lds, threadIdx.x, true);
 }
 
 // Persistent pass kernel for batches: a grid of resident workgroups drains a
@@ -517,7 +530,7 @@ __global__ void __launch_bounds__(MAX_TILE_THREADS, 3) ntt_pass_kernel(PassDesc 
 // per CU (three would spill), so it exchanges all limbs at once and prefetches
 // the twiddles of tiles up to 2^9 elements.
 template <int L>
-using BatchXchg = Xchg<false, (L <= 9)>;
+using BatchXchg = Xchg<false, (L <= 9), false>;
 constexpr uint32_t BATCH_TILE_LOG = 11;  // 2048-element tiles: 72 KiB of LDS
 constexpr int BATCH_THREADS = 1 << (BATCH_TILE_LOG - 3);
 constexpr uint32_t BATCH_ITEM_WORD = (uint32_t)NLIMBS << BATCH_TILE_LOG;  // LDS word index of the item slot
@@ -711,7 +724,14 @@ static uint32_t persistent_lds_bytes() { return 4u * BATCH_ITEM_WORD + 16u; }
 
 template <int LOGN>
 static void launch_one(const PassDesc& d, hipStream_t s) {
-  ntt_pass_kernel<LOGN><<<d.num_tiles, tile_threads(LOGN, d.log_c), tile_lds_bytes(LOGN, d.log_c), s>>>(d);
+  const uint32_t threads = tile_threads(LOGN, d.log_c), bytes = tile_lds_bytes(LOGN, d.log_c);
+  if constexpr (LOGN <= (int)NARROW_TILE_LOG) {
+    if (LOGN + d.log_c <= NARROW_TILE_LOG) {
+      ntt_pass_kernel_narrow<LOGN><<<d.num_tiles, threads, bytes, s>>>(d);
+      return;
+    }
+  }
+  ntt_pass_kernel<LOGN><<<d.num_tiles, threads, bytes, s>>>(d);
 }
 #define ZK_TILE_CASES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12)
 
@@ -806,6 +826,10 @@ hipError_t configure_kernels() {
   if (e == hipSuccess)                                                                                    \
     e = hipFuncSetAttribute((const void*)ntt_pass_kernel<L>, hipFuncAttributeMaxDynamicSharedMemorySize, \
                             max_lds);                                                                     \
+  if (e == hipSuccess && L <= (int)NARROW_TILE_LOG)                                                       \
+    e = hipFuncSetAttribute((const void*)ntt_pass_kernel_narrow<(L <= 11 ? L : 11)>,                      \
+                            hipFuncAttributeMaxDynamicSharedMemorySize,                                   \
+                            (int)(4u * NarrowXchg::words) << NARROW_TILE_LOG);                            \
   ZK_TILE_CASES(X)
 #undef X
   if (e == hipSuccess)
