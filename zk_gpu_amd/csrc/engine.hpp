@@ -195,7 +195,7 @@ hipError_t launch_reset_counters(uint32_t* d_counter, uint32_t n, hipStream_t s)
 uint32_t batch_tile_log();
 hipError_t launch_mad_probe(uint64_t* out, uint32_t blocks, uint32_t seed, hipStream_t s);
 uint64_t mad_probe_ops_per_thread();
-uint32_t tile_threads(uint32_t log_n_tile, uint32_t log_c);
+uint32_t tile_threads(uint32_t log_n_tile, uint32_t log_c, uint32_t le_max);  // threads of a tile
 uint32_t batch_items(const PassDesc& d);  // queue items of a persistent-kernel descriptor
 uint32_t persistent_wgs_per_cu();          // resident persistent-kernel workgroups per CU
 uint32_t tile_lds_bytes(uint32_t log_n_tile, uint32_t log_c);

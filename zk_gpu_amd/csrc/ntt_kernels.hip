@@ -86,11 +86,13 @@ __device__ __forceinline__ void store_planar(void* base, uint32_t plane, uint32_
 //   prefetch:  the round twiddles are loaded before the LDS reads and the
 //              barrier (more registers live across it).
 //   chain:     the multiplier's single-chain column form (mont_mul<true>).
-template <bool TWO_PHASE, bool PREFETCH, bool CHAIN>
+//   le_max:    log2 of the elements each thread keeps (radix of the rounds).
+template <bool TWO_PHASE, bool PREFETCH, bool CHAIN, int LE_MAX = 3>
 struct Xchg {
   static constexpr bool two_phase = TWO_PHASE;
   static constexpr bool prefetch = PREFETCH;
   static constexpr bool chain = CHAIN;
+  static constexpr int le_max = LE_MAX;
   static constexpr uint32_t words = TWO_PHASE ? 5u : (uint32_t)NLIMBS;  // LDS words per tile element
 };
 
@@ -441,19 +443,23 @@ __device__ __forceinline__ void ntt_round(fe (&u)[E], const PassDesc& d, const T
   if (!last) __syncthreads();  // this round's LDS writes precede the next round's reads
 }
 
-// Elements per thread: radix-8 rounds keep 8 x 9 data VGPRs per thread.
-template <int LOGN> struct TileShape {
-  static constexpr int LE = LOGN < 3 ? LOGN : 3;         // log2 elements per thread
+// Elements per thread: radix-8 rounds keep 8 x 9 data VGPRs per thread
+// (LE_MAX = 3); radix-4 rounds keep 4 x 9 (LE_MAX = 2).
+template <int LOGN, int LE_MAX = 3> struct TileShape {
+  static constexpr int LE = LOGN < LE_MAX ? LOGN : LE_MAX;  // log2 elements per thread
   static constexpr int E = 1 << LE;
   static constexpr int FULL = LOGN / LE;                 // rounds of radix E
   static constexpr int REM = LOGN - FULL * LE;           // log2 radix of the last round (0: none)
   static constexpr int ROUNDS = FULL + (REM ? 1 : 0);
 };
 
-template <int LOGN, int RIDX, class X>
-__device__ __forceinline__ void ntt_rounds(fe (&u)[TileShape<LOGN>::E], const PassDesc& d, const TileCtx& t,
-                                           uint32_t* lds) {
-  using Sh = TileShape<LOGN>;
+// the tile shape of exchange configuration X
+template <int LOGN, class X>
+using TileOf = TileShape<LOGN, X::le_max>;
+
+template <int LOGN, int RIDX, class X, int E = TileOf<LOGN, X>::E>
+__device__ __forceinline__ void ntt_rounds(fe (&u)[E], const PassDesc& d, const TileCtx& t, uint32_t* lds) {
+  using Sh = TileOf<LOGN, X>;
   if constexpr (RIDX < Sh::ROUNDS) {
     constexpr int LR = RIDX < Sh::FULL ? Sh::LE : Sh::REM;
     constexpr uint32_t LOGP = (uint32_t)(RIDX * Sh::LE);
@@ -467,7 +473,7 @@ __device__ __forceinline__ void ntt_rounds(fe (&u)[TileShape<LOGN>::E], const Pa
 // still take every barrier.
 template <int LOGN, class X>
 __device__ __forceinline__ void ntt_tile(const PassDesc& d, uint32_t tile, uint32_t* lds, uint32_t tid, bool valid) {
-  using Sh = TileShape<LOGN>;
+  using Sh = TileOf<LOGN, X>;
   TileCtx t;
   t.logN = LOGN;
   t.logC = d.log_c;
@@ -545,7 +551,7 @@ template <int L>
 __device__ __forceinline__ void tile_dispatch(const PassDesc& d, uint32_t group, uint32_t* lds) {
   if constexpr (L <= (int)BATCH_TILE_LOG) {
     if (d.log_n_tile == (uint32_t)L) {
-      const uint32_t lgt = L + d.log_c - TileShape<L>::LE;  // log2 threads per tile (<= BATCH_THREADS_LOG)
+      const uint32_t lgt = L + d.log_c - TileOf<L, BatchXchg<L>>::LE;  // log2 threads per tile
       const uint32_t sub = threadIdx.x >> lgt;
       const uint32_t tile = (group << (BATCH_THREADS_LOG - lgt)) + sub;
       ntt_tile<L, BatchXchg<L>>(d, tile, lds + sub * (BatchXchg<L>::words << (L + d.log_c)), threadIdx.x & ((1u << lgt) - 1u),
@@ -706,8 +712,8 @@ __global__ void check_canonical_kernel(const void* in, uint64_t count, uint32_t*
 
 // ---------------------------------------------------------------- launchers
 
-uint32_t tile_threads(uint32_t log_n_tile, uint32_t log_c) {
-  const uint32_t le = log_n_tile < 3 ? log_n_tile : 3;
+uint32_t tile_threads(uint32_t log_n_tile, uint32_t log_c, uint32_t le_max) {
+  const uint32_t le = log_n_tile < le_max ? log_n_tile : le_max;
   return 1u << (log_n_tile + log_c - le);
 }
 uint32_t tile_lds_bytes(uint32_t log_n_tile, uint32_t log_c) {
@@ -715,7 +721,7 @@ uint32_t tile_lds_bytes(uint32_t log_n_tile, uint32_t log_c) {
 }
 // queue items of a batch descriptor: groups of tiles that fill BATCH_THREADS
 uint32_t batch_items(const PassDesc& d) {
-  const uint32_t per_item = (uint32_t)BATCH_THREADS / tile_threads(d.log_n_tile, d.log_c);
+  const uint32_t per_item = (uint32_t)BATCH_THREADS / tile_threads(d.log_n_tile, d.log_c, 3);
   return (d.num_tiles + per_item - 1) / per_item;
 }
 uint32_t persistent_wgs_per_cu() { return (uint32_t)PERSIST_WGS; }
@@ -724,14 +730,14 @@ static uint32_t persistent_lds_bytes() { return 4u * BATCH_ITEM_WORD + 16u; }
 
 template <int LOGN>
 static void launch_one(const PassDesc& d, hipStream_t s) {
-  const uint32_t threads = tile_threads(LOGN, d.log_c), bytes = tile_lds_bytes(LOGN, d.log_c);
+  const uint32_t bytes = tile_lds_bytes(LOGN, d.log_c);
   if constexpr (LOGN <= (int)NARROW_TILE_LOG) {
     if (LOGN + d.log_c <= NARROW_TILE_LOG) {
-      ntt_pass_kernel_narrow<LOGN><<<d.num_tiles, threads, bytes, s>>>(d);
+      ntt_pass_kernel_narrow<LOGN><<<d.num_tiles, tile_threads(LOGN, d.log_c, NarrowXchg::le_max), bytes, s>>>(d);
       return;
     }
   }
-  ntt_pass_kernel<LOGN><<<d.num_tiles, threads, bytes, s>>>(d);
+  ntt_pass_kernel<LOGN><<<d.num_tiles, tile_threads(LOGN, d.log_c, PassXchg::le_max), bytes, s>>>(d);
 }
 #define ZK_TILE_CASES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12)
 
