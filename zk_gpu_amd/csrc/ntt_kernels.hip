@@ -529,21 +529,23 @@ __global__ void __launch_bounds__(1 << (NARROW_TILE_LOG - 3), 4) ntt_pass_kernel
 // counts (batch_items). Every workgroup leaves the loop once the counter
 // passes `total`.
 //
-// LDS layout: the tiles' three limb planes from byte 0 (16-byte aligned, as
-// in the single-pass kernel: the planes are accessed with 16-byte DS
-// operations), then one word holding the claimed item.
-// The persistent kernel inlines every tile length and runs two workgroups
-// per CU (three would spill), so it exchanges all limbs at once and prefetches
-// the twiddles of tiles up to 2^9 elements.
+// LDS layout: the tiles' limb planes (two-phase exchange) from byte 0
+// (16-byte aligned, as in the single-pass kernel: the planes are accessed with
+// 16-byte DS operations), then one word holding the claimed item.
+//
+// The persistent kernel inlines every tile length. With the two-phase
+// exchange and single-chain columns it needs 149 VGPRs (no spills), so three
+// workgroups share a CU (three waves per SIMD, 3 x 40 KiB of LDS); with the
+// one-phase exchange and split columns it needed 234 and ran two.
 template <int L>
-using BatchXchg = Xchg<false, (L <= 9), false>;
-constexpr uint32_t BATCH_TILE_LOG = 11;  // 2048-element tiles: 72 KiB of LDS
+using BatchXchg = Xchg<true, false, true>;
+constexpr uint32_t BATCH_TILE_LOG = 11;  // 2048-element tiles: 40 KiB of LDS
 constexpr int BATCH_THREADS = 1 << (BATCH_TILE_LOG - 3);
-constexpr uint32_t BATCH_ITEM_WORD = (uint32_t)NLIMBS << BATCH_TILE_LOG;  // LDS word index of the item slot
+constexpr uint32_t BATCH_ITEM_WORD = BatchXchg<BATCH_TILE_LOG>::words << BATCH_TILE_LOG;  // LDS word of the item slot
 
 constexpr uint32_t BATCH_THREADS_LOG = BATCH_TILE_LOG - 3;
 // resident workgroups per CU (one wave per SIMD each)
-constexpr int PERSIST_WGS = 2;
+constexpr int PERSIST_WGS = 3;
 
 // Item `group` of descriptor d: tiles group * G .. group * G + G - 1, G =
 // BATCH_THREADS / (threads per tile).
