@@ -498,14 +498,17 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n) {
 
 constexpr int MAX_TILE_THREADS = 1 << (TILE_LOG_MAX - 3);
 
-// Single-pass tiles, two-phase exchange, no twiddle prefetch, in two
-// families chosen by tile size at launch:
-//   tiles up to 2^11 elements (<= 256 threads): at most 128 VGPRs, four waves
-//     per SIMD (four 40 KiB tiles fill a CU's 160 KiB), single-chain columns;
-//   2^12-element tiles (512 threads): at most 168 VGPRs, three waves per
-//     SIMD, split columns (one such tile per CU).
+// Single-pass tiles without twiddle prefetch, in two families chosen by tile
+// size at launch:
+//   tiles up to 2^11 elements (<= 256 threads): two-phase exchange, at most
+//     128 VGPRs, four waves per SIMD (four 40 KiB tiles fill a CU's
+//     160 KiB), single-chain columns;
+//   2^12-element tiles (512 threads): one such tile per CU whatever its LDS
+//     (a second would need four waves per SIMD, where these tiles spill), so
+//     the one-phase exchange (144 KiB, two barriers per round fewer), at most
+//     168 VGPRs and split columns.
 constexpr uint32_t NARROW_TILE_LOG = 11;
-using PassXchg = Xchg<true, false, false>;
+using PassXchg = Xchg<false, false, false>;
 using NarrowXchg = Xchg<true, false, true>;
 
 template <int LOGN>
@@ -719,7 +722,8 @@ uint32_t tile_threads(uint32_t log_n_tile, uint32_t log_c, uint32_t le_max) {
   return 1u << (log_n_tile + log_c - le);
 }
 uint32_t tile_lds_bytes(uint32_t log_n_tile, uint32_t log_c) {
-  return (uint32_t)(4u * PassXchg::words) << (log_n_tile + log_c);
+  const uint32_t tl = log_n_tile + log_c;
+  return (4u * (tl <= NARROW_TILE_LOG ? NarrowXchg::words : PassXchg::words)) << tl;
 }
 // queue items of a batch descriptor: groups of tiles that fill BATCH_THREADS
 uint32_t batch_items(const PassDesc& d) {
