@@ -505,14 +505,15 @@ constexpr int MAX_TILE_THREADS = 1 << (TILE_LOG_MAX - 3);
 //     160 KiB), single-chain columns;
 //   2^12-element tiles (512 threads): one such tile per CU whatever its LDS
 //     (a second would need four waves per SIMD, where these tiles spill), so
-//     the one-phase exchange (144 KiB, two barriers per round fewer), at most
-//     168 VGPRs and split columns.
+//     the one-phase exchange (144 KiB, two barriers per round fewer), the
+//     two-wave register budget (256 VGPRs; they need 165-174, no spills) and
+//     split columns.
 constexpr uint32_t NARROW_TILE_LOG = 11;
 using PassXchg = Xchg<false, false, false>;
 using NarrowXchg = Xchg<true, false, true>;
 
 template <int LOGN>
-__global__ void __launch_bounds__(MAX_TILE_THREADS, 3) ntt_pass_kernel(PassDesc d) {
+__global__ void __launch_bounds__(MAX_TILE_THREADS, 2) ntt_pass_kernel(PassDesc d) {
   extern __shared__ uint32_t lds[];
   ntt_tile<LOGN, PassXchg>(d, xcd_tile(blockIdx.x, d.num_tiles), lds, threadIdx.x, true);
 }
