@@ -83,14 +83,13 @@ __device__ __forceinline__ void store_planar(void* base, uint32_t plane, uint32_
 //              barrier: 20 B per element, so three 2048-element tiles share a
 //              CU, for two more barriers per exchange. Otherwise all nine
 //              limbs at once in the planes of store_planar (36 B per element).
-//   prefetch:  the round twiddles are loaded before the LDS reads and the
-//              barrier (more registers live across it).
 //   chain:     the multiplier's single-chain column form (mont_mul<true>).
 //   le_max:    log2 of the elements each thread keeps (radix of the rounds).
-template <bool TWO_PHASE, bool PREFETCH, bool CHAIN, int LE_MAX = 3>
+// The round twiddles are loaded after the barriers: loading them ahead,
+// across the barrier, spilled or ran slower in every configuration.
+template <bool TWO_PHASE, bool CHAIN, int LE_MAX = 3>
 struct Xchg {
   static constexpr bool two_phase = TWO_PHASE;
-  static constexpr bool prefetch = PREFETCH;
   static constexpr bool chain = CHAIN;
   static constexpr int le_max = LE_MAX;
   static constexpr uint32_t words = TWO_PHASE ? 5u : (uint32_t)NLIMBS;  // LDS words per tile element
@@ -367,17 +366,7 @@ __device__ __forceinline__ void ntt_round(fe (&u)[E], const PassDesc& d, const T
     auto twiddle = [&](int s, int j) {
       return load_limbs48(roots, (uint64_t)(j * ((t.i + s * NE) & pmask)) << sh);
     };
-    fe tw[X::prefetch ? S : 1][X::prefetch ? R : 1];
-    if (t.active) {
-      if constexpr (X::prefetch) {
-        static_for<S>([&](auto sc) {
-          static_for<R, 1>([&](auto jc) {
-            tw[decltype(sc)::value][decltype(jc)::value] = twiddle(decltype(sc)::value, decltype(jc)::value);
-          });
-        });
-      }
-      lds_read_round<E, R, X::two_phase ? 0 : 2>(u, t, lds);
-    }
+    if (t.active) lds_read_round<E, R, X::two_phase ? 0 : 2>(u, t, lds);
     __syncthreads();  // every LDS read precedes the next overwrite
     if constexpr (X::two_phase) {
       // limbs 5-8: the previous round (radix E, since only a tile's last
@@ -394,8 +383,7 @@ __device__ __forceinline__ void ntt_round(fe (&u)[E], const PassDesc& d, const T
         reduce_value(u[s * R]);
         static_for<R, 1>([&](auto jc) {
           constexpr int j = decltype(jc)::value;
-          if constexpr (X::prefetch) u[s * R + j] = mont_mul<X::chain>(u[s * R + j], tw[s][j]);
-          else u[s * R + j] = mont_mul<X::chain>(u[s * R + j], twiddle(s, j));
+          u[s * R + j] = mont_mul<X::chain>(u[s * R + j], twiddle(s, j));
         });
       });
     }
@@ -498,8 +486,7 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n) {
 
 constexpr int MAX_TILE_THREADS = 1 << (TILE_LOG_MAX - 3);
 
-// Single-pass tiles without twiddle prefetch, in two families chosen by tile
-// size at launch:
+// Single-pass tiles in two families chosen by tile size at launch:
 //   tiles up to 2^11 elements (<= 256 threads): two-phase exchange, at most
 //     128 VGPRs, four waves per SIMD (four 40 KiB tiles fill a CU's
 //     160 KiB), single-chain columns;
@@ -509,8 +496,8 @@ constexpr int MAX_TILE_THREADS = 1 << (TILE_LOG_MAX - 3);
 //     two-wave register budget (256 VGPRs; they need 165-174, no spills) and
 //     split columns.
 constexpr uint32_t NARROW_TILE_LOG = 11;
-using PassXchg = Xchg<false, false, false>;
-using NarrowXchg = Xchg<true, false, true>;
+using PassXchg = Xchg<false, false>;
+using NarrowXchg = Xchg<true, true>;
 
 template <int LOGN>
 __global__ void __launch_bounds__(MAX_TILE_THREADS, 2) ntt_pass_kernel(PassDesc d) {
@@ -541,11 +528,10 @@ __global__ void __launch_bounds__(1 << (NARROW_TILE_LOG - 3), 4) ntt_pass_kernel
 // exchange and single-chain columns it needs 149 VGPRs (no spills), so three
 // workgroups share a CU (three waves per SIMD, 3 x 40 KiB of LDS); with the
 // one-phase exchange and split columns it needed 234 and ran two.
-template <int L>
-using BatchXchg = Xchg<true, false, true>;
+using BatchXchg = Xchg<true, true>;
 constexpr uint32_t BATCH_TILE_LOG = 11;  // 2048-element tiles: 40 KiB of LDS
 constexpr int BATCH_THREADS = 1 << (BATCH_TILE_LOG - 3);
-constexpr uint32_t BATCH_ITEM_WORD = BatchXchg<BATCH_TILE_LOG>::words << BATCH_TILE_LOG;  // LDS word of the item slot
+constexpr uint32_t BATCH_ITEM_WORD = BatchXchg::words << BATCH_TILE_LOG;  // LDS word of the item slot
 
 constexpr uint32_t BATCH_THREADS_LOG = BATCH_TILE_LOG - 3;
 // resident workgroups per CU (one wave per SIMD each)
@@ -557,11 +543,11 @@ template <int L>
 __device__ __forceinline__ void tile_dispatch(const PassDesc& d, uint32_t group, uint32_t* lds) {
   if constexpr (L <= (int)BATCH_TILE_LOG) {
     if (d.log_n_tile == (uint32_t)L) {
-      const uint32_t lgt = L + d.log_c - TileOf<L, BatchXchg<L>>::LE;  // log2 threads per tile
+      const uint32_t lgt = L + d.log_c - TileOf<L, BatchXchg>::LE;  // log2 threads per tile
       const uint32_t sub = threadIdx.x >> lgt;
       const uint32_t tile = (group << (BATCH_THREADS_LOG - lgt)) + sub;
-      ntt_tile<L, BatchXchg<L>>(d, tile, lds + sub * (BatchXchg<L>::words << (L + d.log_c)), threadIdx.x & ((1u << lgt) - 1u),
-                  tile < d.num_tiles);
+      ntt_tile<L, BatchXchg>(d, tile, lds + sub * (BatchXchg::words << (L + d.log_c)),
+                             threadIdx.x & ((1u << lgt) - 1u), tile < d.num_tiles);
       return;
     }
     tile_dispatch<L + 1>(d, group, lds);
