@@ -198,6 +198,5 @@ uint64_t mad_probe_ops_per_thread();
 uint32_t tile_threads(uint32_t log_n_tile, uint32_t log_c, uint32_t le_max);  // threads of a tile
 uint32_t batch_items(const PassDesc& d);  // queue items of a persistent-kernel descriptor
 uint32_t persistent_wgs_per_cu();          // resident persistent-kernel workgroups per CU
-uint32_t tile_lds_bytes(uint32_t log_n_tile, uint32_t log_c);
 
 }  // namespace zkgpu

@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
 #include <type_traits>
 
 #include "fr29.hpp"
@@ -486,15 +487,18 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n) {
 
 constexpr int MAX_TILE_THREADS = 1 << (TILE_LOG_MAX - 3);
 
-// Single-pass tiles in two families chosen by tile size at launch:
-//   tiles up to 2^11 elements (<= 256 threads): two-phase exchange, at most
-//     128 VGPRs, four waves per SIMD (four 40 KiB tiles fill a CU's
-//     160 KiB), single-chain columns;
-//   2^12-element tiles (512 threads): one such tile per CU whatever its LDS
-//     (a second would need four waves per SIMD, where these tiles spill), so
-//     the one-phase exchange (144 KiB, two barriers per round fewer), the
-//     two-wave register budget (256 VGPRs; they need 165-174, no spills) and
-//     split columns.
+// Single-pass tiles in two families chosen at launch:
+//   narrow: tiles up to 2^11 elements (<= 256 threads) in passes with more
+//     than two waves per SIMD: two-phase exchange, at most 128 VGPRs, four
+//     waves per SIMD (four 40 KiB tiles fill a CU's 160 KiB), single-chain
+//     columns;
+//   wide: 2^12-element tiles (512 threads; one per CU whatever its LDS, a
+//     second would need four waves per SIMD, where these tiles spill), and
+//     the tiles of passes too small to give a SIMD more than two waves (any
+//     transform below 2^21 elements): the one-phase exchange (two barriers
+//     per round fewer), the two-wave register budget (256 VGPRs; they need
+//     165-174, no spills) and split columns, whose independent chains hide
+//     latency that two waves cannot.
 constexpr uint32_t NARROW_TILE_LOG = 11;
 using PassXchg = Xchg<false, false>;
 using NarrowXchg = Xchg<true, true>;
@@ -708,10 +712,9 @@ uint32_t tile_threads(uint32_t log_n_tile, uint32_t log_c, uint32_t le_max) {
   const uint32_t le = log_n_tile < le_max ? log_n_tile : le_max;
   return 1u << (log_n_tile + log_c - le);
 }
-uint32_t tile_lds_bytes(uint32_t log_n_tile, uint32_t log_c) {
-  const uint32_t tl = log_n_tile + log_c;
-  return (4u * (tl <= NARROW_TILE_LOG ? NarrowXchg::words : PassXchg::words)) << tl;
-}
+// a pass runs the narrow family when it has more waves than this (two per
+// SIMD of the device; set by configure_kernels)
+static std::atomic<uint64_t> g_narrow_min_waves{2048};
 // queue items of a batch descriptor: groups of tiles that fill BATCH_THREADS
 uint32_t batch_items(const PassDesc& d) {
   const uint32_t per_item = (uint32_t)BATCH_THREADS / tile_threads(d.log_n_tile, d.log_c, 3);
@@ -723,14 +726,17 @@ static uint32_t persistent_lds_bytes() { return 4u * BATCH_ITEM_WORD + 16u; }
 
 template <int LOGN>
 static void launch_one(const PassDesc& d, hipStream_t s) {
-  const uint32_t bytes = tile_lds_bytes(LOGN, d.log_c);
+  const uint32_t tl = LOGN + d.log_c;
   if constexpr (LOGN <= (int)NARROW_TILE_LOG) {
-    if (LOGN + d.log_c <= NARROW_TILE_LOG) {
-      ntt_pass_kernel_narrow<LOGN><<<d.num_tiles, tile_threads(LOGN, d.log_c, NarrowXchg::le_max), bytes, s>>>(d);
+    const uint32_t threads = tile_threads(LOGN, d.log_c, NarrowXchg::le_max);
+    const uint64_t waves = ((uint64_t)d.num_tiles * threads + 63) / 64;
+    if (tl <= NARROW_TILE_LOG && waves > g_narrow_min_waves.load(std::memory_order_relaxed)) {
+      ntt_pass_kernel_narrow<LOGN><<<d.num_tiles, threads, (4u * NarrowXchg::words) << tl, s>>>(d);
       return;
     }
   }
-  ntt_pass_kernel<LOGN><<<d.num_tiles, tile_threads(LOGN, d.log_c, PassXchg::le_max), bytes, s>>>(d);
+  ntt_pass_kernel<LOGN><<<d.num_tiles, tile_threads(LOGN, d.log_c, PassXchg::le_max),
+                          (4u * PassXchg::words) << tl, s>>>(d);
 }
 #define ZK_TILE_CASES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12)
 
@@ -820,7 +826,10 @@ hipError_t launch_mad_probe(uint64_t* out, uint32_t blocks, uint32_t seed, hipSt
 
 hipError_t configure_kernels() {
   const int max_lds = (int)(4u * PassXchg::words) << TILE_LOG_MAX;
-  hipError_t e = hipSuccess;
+  int dev = 0, cus = 256;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess) g_narrow_min_waves.store(8ull * (uint64_t)cus, std::memory_order_relaxed);  // 2 x 4 SIMDs
 #define X(L)                                                                                              \
   if (e == hipSuccess)                                                                                    \
     e = hipFuncSetAttribute((const void*)ntt_pass_kernel<L>, hipFuncAttributeMaxDynamicSharedMemorySize, \
