@@ -81,9 +81,10 @@ __device__ __forceinline__ void store_planar(void* base, uint32_t plane, uint32_
 // How the rounds of a tile exchange data through LDS.
 //   two_phase: limbs 0-3 (a uint4 plane) and limb 4 (a u32 plane), then
 //              limbs 5-8 in the same uint4 plane, each phase closed by a
-//              barrier: 20 B per element, so three 2048-element tiles share a
-//              CU, for two more barriers per exchange. Otherwise all nine
-//              limbs at once in the planes of store_planar (36 B per element).
+//              barrier: 20 B per element (a 2048-element tile takes 40 KiB
+//              instead of 72 KiB, so up to four share a CU), for two more
+//              barriers per exchange. Otherwise all nine limbs at once in the
+//              planes of store_planar (36 B per element).
 //   chain:     the multiplier's single-chain column form (mont_mul<true>).
 //   le_max:    log2 of the elements each thread keeps (radix of the rounds).
 // The round twiddles are loaded after the barriers: loading them ahead,
@@ -291,9 +292,10 @@ struct TileCtx {
   bool active;
 };
 
-// Two-phase exchange helpers. A round of radix R = 2^LR at stride 2^logp
-// writes its register s*R + bitrev(m) to tile position base(s) + m*2^logp;
-// the next round reads its register s*R + j from position iv(s) + j*N/R.
+// LDS exchange helpers (PH as in lds_put). A round of radix R = 2^LR at
+// stride 2^logp writes its register s*R + bitrev(m) to tile position
+// base(s) + m*2^logp; the next round reads its register s*R + j from
+// position iv(s) + j*N/R.
 template <int E, int R, int PH>
 __device__ __forceinline__ void lds_write_round(const fe (&u)[E], const TileCtx& t, uint32_t logp, uint32_t* lds) {
   constexpr int S = E / R;
