@@ -42,19 +42,24 @@ def modmuls(log_n, op):
     return m
 
 
-def profiled_traffic():
-    """HBM bytes per k=20 pass launch (rocprofv3 FETCH_SIZE + WRITE_SIZE,
-    calibrated against a copy of known size), from the committed summary in
-    profiles/; None when it is absent."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "ntt_pass_kernel_k20_traffic.txt")
+def profiled_traffic(kernel, log_n):
+    """HBM bytes per launch of `kernel` at 2^log_n (rocprofv3 FETCH_SIZE +
+    WRITE_SIZE, calibrated against a copy of known size), from the committed
+    traffic summaries in profiles/; None when absent."""
+    d = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles")
+    pat = re.compile(r"TRAFFIC kernel=(\S+) k=(\d+) bytes_per_launch=(\d+)")
     try:
-        with open(path) as f:
-            for line in f:
-                m = re.match(r"TRAFFIC k=20 bytes_per_launch=(\d+)", line)
-                if m:
-                    return int(m.group(1))
+        names = sorted(os.listdir(d))
     except OSError:
-        pass
+        return None
+    for fn in names:
+        if not fn.endswith(".txt"):
+            continue
+        with open(os.path.join(d, fn)) as f:
+            for line in f:
+                m = pat.match(line.strip())
+                if m and m.group(1) == kernel and int(m.group(2)) == log_n:
+                    return int(m.group(3))
     return None
 
 
@@ -80,15 +85,27 @@ def passes(Z, log_n):
     return npass, digits, cols
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(budget_s=15.0):
-    """halo2 best_fft restated in C (oracle/), timed on the host cores."""
+    """halo2 best_fft restated in C (oracle/), timed on every host core this
+    process may run on (halo2's log_split = floor(log2 threads)), and on one
+    thread."""
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     from oracle import oracle as O  # the checker, used here only as the timed CPU path
     try:
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
         cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 16))
     k = 20
     x = O.gen(2001, 1 << k)
     ops = ("fwd", "inv", "coset_fwd")
@@ -107,10 +124,52 @@ def cpu_baseline(budget_s=15.0):
     for op in ops:
         O.run_op(op, x, k, threads=1)
     value_1 = len(ops) * (1 << k) / (time.perf_counter() - t0)
-    return {"value": value, "unit": "elem/s", "cores": cores, "kind": "port",
+    return {"value": value, "unit": "elem/s", "cores": cores, "kind": "port", "cpu_model": cpu_model(),
+            "nproc": os.cpu_count(),
             "sample": f"{steps} steps of fwd+inv+coset_fwd at 2^20 (halo2 best_fft restated in C, "
-                      f"log_split=floor(log2 {cores}))",
+                      f"{cores} threads, log_split=floor(log2 {cores}))",
             "value_1_thread": value_1, "sample_1_thread": "1 step of fwd+inv+coset_fwd at 2^20, 1 thread"}
+
+
+def pass_work(log_n, digits, op_names):
+    """Algorithmic Montgomery products of each pass of one step, summed over
+    the step's transforms: (n/2) * digit for the pass's radix-2 butterflies,
+    plus the coset pre-scale (first pass), the inverse's divisor and the
+    coset-inverse post-scale (last pass)."""
+    n = 1 << log_n
+    work = [0] * len(digits)
+    for op in op_names:
+        for i, d in enumerate(digits):
+            work[i] += (n // 2) * d
+        if op in ("coset_fwd",):
+            work[0] += n
+        if op in ("inv", "coset_inv"):
+            work[-1] += n
+        if op == "coset_inv":
+            work[-1] += n
+    return work
+
+
+def kernel_roofline(Z, times, log_n, digits, op_names, p_mul):
+    """Per-kernel achieved multiply-add rate from the pass timing (events tied
+    to each pass dispatch), and the dominant kernel's figures."""
+    names = {}
+    work = pass_work(log_n, digits, op_names)
+    # which kernel ran pass i: its tile digit (equal digits share a kernel)
+    per_kernel = {}
+    for name, (launches, total_ms) in times.items():
+        d = int(name.split("<")[1].rstrip(">"))
+        mm = sum(w for w, dd in zip(work, digits) if dd == d)
+        passes_with_d = sum(1 for dd in digits if dd == d)
+        per_kernel[name] = {"launches": launches, "avg_us": total_ms / launches * 1e3,
+                            "mads_per_launch": mm * MADS_PER_MODMUL / (len(op_names) * passes_with_d)}
+    for name, v in per_kernel.items():
+        v["achieved_tmad_s"] = v["mads_per_launch"] / (v["avg_us"] * 1e-6) / 1e12
+        v["frac"] = v["achieved_tmad_s"] / PEAK_MAD_TOPS
+        v["frac_measured"] = v["achieved_tmad_s"] / p_mul
+        names[name] = v
+    dom = max(names, key=lambda k: names[k]["avg_us"] * names[k]["launches"])
+    return dom, names
 
 
 def bench_batched(torch, Z, dev, stream, count=64, seed=4, reps=10):
@@ -457,12 +516,22 @@ def main():
     launches = 3 * npass
     launch_ms = ms / launches
     mm_step = sum(modmuls(k, name) for name, _ in ops)
-    mads_per_launch = mm_step * MADS_PER_MODMUL / launches
-    achieved = mads_per_launch / (launch_ms * 1e-3) / 1e12
+    # the same steps again with start/stop events tied to every pass dispatch
+    # (hipExtLaunchKernel, no extra packets): per-kernel durations
+    Z.pass_timing(True)
+    Z.pass_times()
+    e0.record(stream)
+    for _ in range(args.steps):
+        step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms_timed = e0.elapsed_time(e1) / args.steps
+    times = Z.pass_times()
+    Z.pass_timing(False)
+    dom, kern = kernel_roofline(Z, times, k, digits, [o for o, _ in ops], p_mul)
+    kd = kern[dom]
     bytes_per_launch = 64 * n
-    hbm = bytes_per_launch / (launch_ms * 1e-3) / 1e9
-    kernel_name = f"ntt_pass_kernel<{digits[0]}>" if len(set(digits)) == 1 else \
-        "ntt_pass_kernel<" + ",".join(str(d) for d in digits) + ">"
+    hbm = bytes_per_launch / (kd["avg_us"] * 1e-6) / 1e9
 
     res = {
         "metric": "BN256 Fr NTT elem/s at k=20/24; served mixed-size vs plain; p50/p99 lat",
@@ -480,16 +549,23 @@ def main():
         "config": {"workload": "fwd+inv+coset_fwd NTT over BN256 Fr at n=2^20 (one step = 3 transforms)",
                    "log_n": k, "ops": [o for o, _ in ops], "coset_g": 7,
                    "plan": {"digits": digits, "log_cols": cols}},
-        "roofline": {"bound": "valu", "kernel": kernel_name, "achieved": achieved, "peak": PEAK_MAD_TOPS,
-                     "unit": "Tmad/s", "frac": achieved / PEAK_MAD_TOPS, "traffic": profiled_traffic(),
-                     "peak_measured": p_mul, "frac_measured": achieved / p_mul,
-                     "launch_ms": launch_ms, "launches_per_step": launches,
-                     "work_per_launch": f"{mads_per_launch:.4g} 32x32->64 multiply-adds "
-                                        f"({mm_step / launches:.4g} Montgomery products x 136)"},
-        "roofline_hbm": {"bound": "hbm", "achieved": hbm, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+        "roofline": {"bound": "valu", "kernel": dom, "achieved": kd["achieved_tmad_s"], "peak": PEAK_MAD_TOPS,
+                     "unit": "Tmad/s", "frac": kd["frac"], "traffic": profiled_traffic(dom, k),
+                     "peak_measured": p_mul, "frac_measured": kd["frac_measured"],
+                     "avg_launch_us": kd["avg_us"], "launches_timed": kd["launches"],
+                     "work_per_launch": f"{kd['mads_per_launch']:.4g} 32x32->64 multiply-adds "
+                                        f"({kd['mads_per_launch'] / MADS_PER_MODMUL:.4g} Montgomery products x 136)",
+                     "timing": "start/stop events tied to each pass dispatch on the bench stream over "
+                               f"{args.steps} steps ({ms_timed:.4f} ms per step with the events, "
+                               f"{ms:.4f} ms without)",
+                     "kernels": kern,
+                     "step_average": {"achieved": mm_step * MADS_PER_MODMUL / launches / (launch_ms * 1e-3) / 1e12,
+                                      "launch_ms": launch_ms}},
+        "roofline_hbm": {"bound": "hbm", "kernel": dom, "achieved": hbm, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                          "frac": hbm / PEAK_HBM_GBS, "bytes_per_launch": bytes_per_launch},
         "wall_s_timed_region": wall,
     }
+    res["roofline"]["step_average"]["frac"] = res["roofline"]["step_average"]["achieved"] / PEAK_MAD_TOPS
 
     if not args.no_k24:
         k24 = 24
@@ -512,17 +588,27 @@ def main():
             runs.append(e0.elapsed_time(e1) / reps)
         ms24 = sorted(runs)[2]
         np24, d24, c24 = passes(Z, k24)
-        l_ms = ms24 / np24
-        mads = modmuls(k24, "fwd") * MADS_PER_MODMUL / np24
-        ach = mads / (l_ms * 1e-3) / 1e12
+        Z.pass_timing(True)
+        Z.pass_times()
+        for _ in range(reps):
+            Z.ntt_std_device(x24.data_ptr(), y24.data_ptr(), k24, Z.OP_FWD, stream=sptr)
+        torch.cuda.synchronize()
+        t24 = Z.pass_times()
+        Z.pass_timing(False)
+        dom24, kern24 = kernel_roofline(Z, t24, k24, d24, ["fwd"], p_mul)
+        kd24 = kern24[dom24]
         res["k24"] = {"op": "fwd", "ms": ms24, "elem_per_s": n24 / (ms24 * 1e-3),
                       "plan": {"digits": d24, "log_cols": c24},
-                      "roofline": {"bound": "valu", "achieved": ach, "peak": PEAK_MAD_TOPS, "unit": "Tmad/s",
-                                   "frac": ach / PEAK_MAD_TOPS, "peak_measured": p_mul, "frac_measured": ach / p_mul,
-                                   "launch_ms": l_ms},
-                      "roofline_hbm": {"bound": "hbm", "achieved": 64 * n24 / (l_ms * 1e-3) / 1e9,
+                      "roofline": {"bound": "valu", "kernel": dom24, "achieved": kd24["achieved_tmad_s"],
+                                   "peak": PEAK_MAD_TOPS, "unit": "Tmad/s", "frac": kd24["frac"],
+                                   "traffic": profiled_traffic(dom24, k24),
+                                   "peak_measured": p_mul, "frac_measured": kd24["frac_measured"],
+                                   "avg_launch_us": kd24["avg_us"], "kernels": kern24,
+                                   "whole_transform_frac": modmuls(k24, "fwd") * MADS_PER_MODMUL /
+                                                           (ms24 * 1e-3) / 1e12 / PEAK_MAD_TOPS},
+                      "roofline_hbm": {"bound": "hbm", "achieved": 64 * n24 / (kd24["avg_us"] * 1e-6) / 1e9,
                                        "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                                       "frac": 64 * n24 / (l_ms * 1e-3) / 1e9 / PEAK_HBM_GBS}}
+                                       "frac": 64 * n24 / (kd24["avg_us"] * 1e-6) / 1e9 / PEAK_HBM_GBS}}
         del x24, y24
 
     if not args.no_batch:

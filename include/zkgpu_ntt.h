@@ -28,7 +28,7 @@ extern "C" {
 #define ZKGPU_ERR_UNSUPPORTED 2  /* entry point not provided (MSM) */
 #define ZKGPU_ERR_HIP 3          /* HIP runtime error */
 #define ZKGPU_ERR_NOMEM 4        /* device allocation failed */
-#define ZKGPU_ERR_NONCANONICAL 5 /* an input element is >= r (validation enabled) */
+#define ZKGPU_ERR_NONCANONICAL 5 /* an input element is >= r (host-buffer entry points, validation on) */
 #define ZKGPU_ERR_BUDGET 6       /* a job exceeds the serving HBM budget */
 #define ZKGPU_ERR_SHUTDOWN 7     /* the server is stopping */
 
@@ -202,6 +202,8 @@ typedef struct zkgpu_server_result {
   uint64_t t_admit_ns;  /* launch of one of its passes, completion seen by */
   uint64_t t_start_ns;  /* the scheduler */
   uint64_t t_done_ns;
+  uint32_t iterations;    /* server iterations (launches) that ran a pass of the job */
+  uint32_t iter_jobs_max; /* most jobs that shared one of those iterations */
 } zkgpu_server_result;
 
 typedef struct zkgpu_server_stats {
@@ -243,6 +245,48 @@ int zkgpu_drop_tables(int device_id);
  * of every transform kernel are made of this instruction; this is the peak
  * the VALU roofline is measured against. */
 int zkgpu_probe_mad_rate(int device_id, double* mads_per_s);
+
+/* Input validation of the host-buffer entry points (parallel_*_api,
+ * zkgpu_coset_ifft_api): when on, the uploaded elements are checked on the
+ * device before the transform, and a call whose input holds an element >= r
+ * returns ZKGPU_ERR_NONCANONICAL with the caller's buffer left unchanged. The
+ * reference does no validation (cuda/fft.cu:275-405). Off by default; on when
+ * the environment sets ZKGPU_VALIDATE=1 or after zkgpu_set_validation(1).
+ * Returns the previous setting. The device-resident entry points never
+ * validate (they are asynchronous). */
+int zkgpu_set_validation(int enable);
+
+/* HBM bytes the host-buffer entry points hold on a device: device buffers of
+ * the fixed pool of host-call contexts (ZKGPU_HOST_STREAMS contexts, 4 by
+ * default, shared by every caller thread) under the ZKGPU_HOST_BUDGET_MB
+ * budget (32 GiB by default; a call larger than the whole budget runs alone).
+ * *peak_bytes (optional) gets the most held since the budget was last set. */
+uint64_t zkgpu_host_held_bytes(int device_id, uint64_t* peak_bytes);
+/* Set that budget (bytes; 0 restores the default); idle contexts release
+ * their buffers down to it and calls that would pass it wait for running
+ * ones, as the reference's GpuLock makes FFT calls wait for GPU memory
+ * (src/gpulock.rs:37-44,133-203). Returns the previous budget. */
+uint64_t zkgpu_set_host_budget(int device_id, uint64_t bytes);
+
+/* Test hook: after `skip` more successful device allocations by the library,
+ * the next `count` fail as if the device were out of memory. */
+int zkgpu_debug_fail_allocs(uint32_t skip, uint32_t count);
+
+/* Per-kernel timing of the pass kernels, for measurement: while enabled,
+ * every pass that a single transform (zkgpu_ntt_device and the entry points
+ * built on it) launches carries a start and a stop event tied to its dispatch
+ * on the caller's stream (hipExtLaunchKernel). zkgpu_pass_times waits for
+ * the recorded launches and reports, per kernel, the launch count and the
+ * summed duration, then forgets them; *count gets the number of kernels
+ * (up to max_out entries are written). */
+typedef struct zkgpu_kernel_time {
+  char name[48];      /* e.g. "ntt_pass_kernel<11>" */
+  uint32_t launches;
+  uint32_t reserved;
+  double total_ms;
+} zkgpu_kernel_time;
+int zkgpu_pass_timing(int device_id, int enable);
+int zkgpu_pass_times(int device_id, zkgpu_kernel_time* out, uint32_t max_out, uint32_t* count);
 
 /* Text of a return code. */
 const char* zkgpu_error_string(int code);

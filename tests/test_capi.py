@@ -123,6 +123,21 @@ def test_inconsistent_plans_rejected():
         Z.plan_set(20, [9, 11], [0, 1], kind=Z.PLAN_BATCH)  # 2^12-element tile in a batch
 
 
+def test_plans_over_workgroup_threads_rejected():
+    """A tile of 1- or 2-bit digits keeps fewer than 8 elements per thread, so
+    its thread count can pass the workgroup size even when its element count
+    fits: 2^1 x 2^10 columns needs 1024 threads (512 per single-pass tile,
+    256 per batch tile). Such splits are refused, not launched."""
+    with pytest.raises(Z.ZkGpuError):
+        Z.plan_set(12, [1, 1, 10], [10, 0, 0], kind=Z.PLAN_BATCH)
+    with pytest.raises(Z.ZkGpuError):
+        Z.plan_set(13, [1, 12], [10, 0], kind=Z.PLAN_SINGLE)
+    with pytest.raises(Z.ZkGpuError):
+        Z.plan_set(13, [2, 11], [9, 0], kind=Z.PLAN_BATCH)
+    Z.plan_set(13, [2, 11], [9, 0], kind=Z.PLAN_SINGLE)  # 512 threads: allowed for single-pass tiles
+    Z.plan_reset(13, kind=Z.PLAN_SINGLE)
+
+
 def test_drop_in_size_checks_before_any_device_work():
     req = np.zeros((1 << 10, 4), dtype=np.uint64)
     w = O.root_of_unity(10)
@@ -140,5 +155,5 @@ def test_job_struct_layout():
     """zkgpu_job as the header lays it out (C and Python sides agree)."""
     assert ctypes.sizeof(Z.Job) == 8 + 8 + 4 * 4 + 3 * 32
     assert Z.Job.omega.offset == 32
-    assert ctypes.sizeof(Z.ServerResult) == 4 + 4 + 8 + 4 * 8
+    assert ctypes.sizeof(Z.ServerResult) == 4 + 4 + 8 + 4 * 8 + 4 + 4
     assert ctypes.sizeof(Z.ServerStats) == 11 * 8

@@ -99,3 +99,21 @@ def test_round_trip_large(torch_cuda, k):
         Z.ntt_std_device(ptr(b), ptr(b), k, inv)
         torch.cuda.synchronize()
         assert torch.equal(a, b), (k, fwd)
+
+
+@pytest.mark.parametrize("op", OPS)
+def test_full_vector_k24_vs_oracle(torch_cuda, op):
+    """The 2^24 configuration: every one of the 2^24 outputs of each op equals
+    the oracle on the same seeded input."""
+    torch = torch_cuda
+    k = 24
+    n = 1 << k
+    x = O.gen(2424, n)
+    d = to_dev(torch, x)
+    Z.ntt_std_device(ptr(d), ptr(d), k, Z.OPS[op])
+    torch.cuda.synchronize()
+    got = to_host(d)
+    del d
+    want = O.run_op(op, x, k, threads=16)
+    mism = np.nonzero((got != want).any(axis=1))[0]
+    assert mism.size == 0, (op, mism[:8])

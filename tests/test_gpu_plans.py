@@ -47,9 +47,10 @@ def test_all_candidate_splits(torch_cuda, k, kind):
 
 @pytest.mark.parametrize("k", list(range(12, 24)))
 def test_tuned_plans_vs_oracle(torch_cuda, k):
-    """The split each plan file holds for 2^k (whatever the tuner chose),
-    single-transform table through the pass kernels and batch table through
-    the persistent kernel, against the oracle."""
+    """The split each plan file holds for 2^k (whatever the tuner chose):
+    the single-transform table through the pass kernels, and the batch table
+    through the persistent kernel (a one-job Batch always runs it; so does a
+    server iteration holding two jobs), against the oracle."""
     torch = torch_cuda
     n = 1 << k
     x = O.gen(1300 + k, n)
@@ -61,7 +62,21 @@ def test_tuned_plans_vs_oracle(torch_cuda, k):
         torch.cuda.synchronize()
         assert torch.equal(d_out, want), ("single", op, Z.plan_get(k, Z.PLAN_SINGLE))
         if k <= 22:
-            srv = Z.Server()
-            srv.wait(srv.submit(Z.make_job(ptr(d_in), ptr(d_out), k, Z.OPS[op])))
-            srv.close()
+            d_out.zero_()
+            b = Z.Batch([Z.make_job(ptr(d_in), ptr(d_out), k, Z.OPS[op])])
+            b.launch()
+            torch.cuda.synchronize()
+            b.close()
             assert torch.equal(d_out, want), ("batch", op, Z.plan_get(k, Z.PLAN_BATCH))
+            # two jobs submitted together share the server's iterations
+            d_out2 = dev_empty(torch, n)
+            d_out.zero_()
+            srv = Z.Server()
+            try:
+                t1 = srv.submit(Z.make_job(ptr(d_in), ptr(d_out), k, Z.OPS[op]))
+                t2 = srv.submit(Z.make_job(ptr(d_in), ptr(d_out2), k, Z.OPS[op]))
+                r1, r2 = srv.wait(t1), srv.wait(t2)
+            finally:
+                srv.close()
+            assert torch.equal(d_out, want) and torch.equal(d_out2, want), ("served", op)
+            assert max(r1.iter_jobs_max, r2.iter_jobs_max) >= 1

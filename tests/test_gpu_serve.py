@@ -149,3 +149,93 @@ def test_serve_concurrent_submitters(torch_cuda):
         assert srv.stats()["completed"] == 40
     finally:
         srv.close()
+
+
+def test_serve_c5_poisson_stream(torch_cuda):
+    """The serving configuration at test scale: 1,000 jobs arriving as a
+    seeded Poisson stream (about 8,000 jobs/s, near the server's capacity, so
+    most iterations hold several jobs), k uniform in 14..22, ops uniform over
+    the four, HBM budget 64 GiB, results in server-owned buffers (every fifth
+    in a caller buffer). Every result equals the same job run alone; every job
+    with k <= 16, and eight larger ones (four at k = 22), equal the oracle."""
+    import threading
+    torch = torch_cuda
+    count = 1000
+    rnd = random.Random(2025)
+    spec = [(rnd.randint(14, 22), rnd.choice(OPNAMES)) for _ in range(count)]
+    seeds = [7000, 7001, 7002, 7003]
+    src = []
+    for s_ in seeds:  # job i reads the first 2^k elements of source i % 4
+        t = dev_empty(torch, 1 << 22)
+        Z.fill_random(ptr(t), 1 << 22, s_)
+        src.append(t)
+    caller_out = {i: dev_empty(torch, 1 << k) for i, (k, _) in enumerate(spec) if i % 5 == 0}
+    torch.cuda.synchronize()
+    t_arr, arr = 0.0, []
+    for _ in range(count):
+        t_arr += rnd.expovariate(8000.0)
+        arr.append(t_arr)
+    large = [i for i, (k, _) in enumerate(spec) if k == 22][:4] + \
+            [i for i, (k, _) in enumerate(spec) if 17 <= k <= 21][:4]
+    assert len(large) == 8
+    srv = Z.Server(budget_bytes=64 << 30)
+    tickets = [None] * count
+    sub_err = []
+
+    def submitter():
+        try:
+            t0 = time.monotonic()
+            for i, ((k, op), a) in enumerate(zip(spec, arr)):
+                d = t0 + a - time.monotonic()
+                if d > 0:
+                    time.sleep(d)
+                o = caller_out.get(i)
+                tickets[i] = srv.submit(Z.make_job(ptr(src[i % 4]), ptr(o) if o is not None else None, k, Z.OPS[op]))
+        except Exception as e:  # reported below
+            sub_err.append(repr(e))
+
+    keep = {}
+    mismatches, shared = [], 0
+    alone = {k: dev_empty(torch, 1 << k) for k in range(14, 23)}
+    th = threading.Thread(target=submitter)
+    try:
+        th.start()
+        for i, (k, op) in enumerate(spec):
+            while tickets[i] is None and not sub_err:
+                time.sleep(0.0002)
+            assert not sub_err, sub_err
+            r = srv.wait(tickets[i])
+            assert r.done == 1 and r.status == 0
+            shared += int(r.iter_jobs_max >= 2)
+            got = caller_out[i] if i in caller_out else None
+            if got is None:
+                got = dev_empty(torch, 1 << k)
+                import ctypes
+                hip = ctypes.CDLL("libamdhip64.so")
+                assert hip.hipMemcpy(ctypes.c_void_p(ptr(got)), ctypes.c_void_p(r.d_out),
+                                     ctypes.c_size_t(32 << k), 3) == 0
+            Z.ntt_std_device(ptr(src[i % 4]), ptr(alone[k]), k, Z.OPS[op])
+            torch.cuda.synchronize()
+            if not torch.equal(got, alone[k]):
+                mismatches.append((i, k, op))
+            if k <= 16 or i in large:
+                keep[i] = to_host(got)
+            srv.release(tickets[i])
+            caller_out.pop(i, None)
+        th.join()
+        st = srv.stats()
+    finally:
+        srv.close()
+    assert not mismatches, mismatches[:10]
+    assert st["completed"] == count and st["failed"] == 0
+    assert st["peak_in_use_bytes"] <= st["budget_bytes"]
+    assert shared >= count // 2, shared  # continuous batching: most jobs shared an iteration
+    xs = {s_: O.gen(s_, 1 << 22) for s_ in seeds}
+    bad = []
+    for i, got in keep.items():
+        k, op = spec[i]
+        want = O.run_op(op, np.ascontiguousarray(xs[seeds[i % 4]][:1 << k]), k, threads=16)
+        if not (got == want).all():
+            bad.append((i, k, op))
+    assert not bad, bad
+    assert sum(1 for i in keep if spec[i][0] == 22) == 4

@@ -75,6 +75,12 @@ SIGNATURES = {
     "zkgpu_error_string": (ctypes.c_char_p, [_I]),
     "zkgpu_probe_mad_rate": (_I, [_I, _P]),
     "zkgpu_version": (_I, []),
+    "zkgpu_set_validation": (_I, [_I]),
+    "zkgpu_host_held_bytes": (_U64, [_I, _P]),
+    "zkgpu_set_host_budget": (_U64, [_I, _U64]),
+    "zkgpu_debug_fail_allocs": (_I, [_U32, _U32]),
+    "zkgpu_pass_timing": (_I, [_I, _I]),
+    "zkgpu_pass_times": (_I, [_I, _P, _U32, _P]),
 }
 OPTIONAL = {}
 
@@ -298,7 +304,8 @@ class ServerResult(ctypes.Structure):
     """zkgpu_server_result (include/zkgpu_ntt.h)."""
     _fields_ = [("status", ctypes.c_int32), ("done", ctypes.c_int32), ("d_out", ctypes.c_void_p),
                 ("t_submit_ns", ctypes.c_uint64), ("t_admit_ns", ctypes.c_uint64),
-                ("t_start_ns", ctypes.c_uint64), ("t_done_ns", ctypes.c_uint64)]
+                ("t_start_ns", ctypes.c_uint64), ("t_done_ns", ctypes.c_uint64),
+                ("iterations", ctypes.c_uint32), ("iter_jobs_max", ctypes.c_uint32)]
 
 
 class ServerStats(ctypes.Structure):
@@ -358,6 +365,25 @@ def probe_mad_rate(device_id=0):
     v = ctypes.c_double(0)
     check(lib().zkgpu_probe_mad_rate(device_id, ctypes.byref(v)), "zkgpu_probe_mad_rate")
     return v.value
+
+
+class KernelTime(ctypes.Structure):
+    """zkgpu_kernel_time (include/zkgpu_ntt.h)."""
+    _fields_ = [("name", ctypes.c_char * 48), ("launches", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("total_ms", ctypes.c_double)]
+
+
+def pass_timing(enable, device_id=0):
+    """Turn per-pass kernel timing (events tied to each pass dispatch) on or off."""
+    check(lib().zkgpu_pass_timing(device_id, 1 if enable else 0), "zkgpu_pass_timing")
+
+
+def pass_times(device_id=0):
+    """{kernel name: (launches, total ms)} of the passes timed since the last read."""
+    buf = (KernelTime * 32)()
+    cnt = ctypes.c_uint32(0)
+    check(lib().zkgpu_pass_times(device_id, buf, 32, ctypes.byref(cnt)), "zkgpu_pass_times")
+    return {buf[i].name.decode(): (buf[i].launches, buf[i].total_ms) for i in range(min(cnt.value, 32))}
 
 
 def version():

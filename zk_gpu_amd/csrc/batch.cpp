@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "../../include/zkgpu_ntt.h"
@@ -40,11 +41,21 @@ struct Batch {
   uint32_t* d_counters = nullptr;
   PassDesc* d_descs = nullptr;
   uint32_t* d_tile_start = nullptr;
+  WsLease arena_lease, meta_lease;  // engine workspaces of an uncaptured batch
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipStream_t capture_stream = nullptr;
   uint64_t elements = 0;
+  // an event per stream the batch was launched on, recorded after each launch
+  std::mutex launch_mu;
+  std::vector<std::pair<hipStream_t, hipEvent_t>> launched;
+  // wait for every launch of the batch (only the streams it ran on)
+  void wait_launches() {
+    std::lock_guard<std::mutex> g(launch_mu);
+    for (auto& kv : launched) (void)hipEventSynchronize(kv.second);
+  }
   ~Batch() {
+    for (auto& kv : launched) (void)hipEventDestroy(kv.second);
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     if (capture_stream) (void)hipStreamDestroy(capture_stream);
@@ -73,11 +84,10 @@ int batch_build(int device, const zkgpu_job* cjobs, uint32_t count, Batch** out,
   b->device = device;
   b->jobs.resize(count);
   b->tables.resize(count);
-  hipStream_t s = e->thread_stream();
   for (uint32_t i = 0; i < count; ++i) {
     st = job_from_c(cjobs[i], b->jobs[i], false);
     if (st) return st;
-    st = e->prepare(b->jobs[i], b->tables[i], s, PLAN_BATCH);
+    st = e->prepare(b->jobs[i], b->tables[i], nullptr, PLAN_BATCH);
     if (st) return st;
     b->elements += 1ull << b->jobs[i].log_n;
   }
@@ -91,8 +101,9 @@ int batch_build(int device, const zkgpu_job* cjobs, uint32_t count, Batch** out,
     }
   }
   if (arena_bytes && opt.use_ws) {
-    b->arena_ptr = e->workspace(opt.ws_stream, arena_bytes, WS_BATCH_ARENA);
-    if (!b->arena_ptr) return ZK_ERR_NOMEM;
+    b->arena_lease = e->lease_workspace(opt.ws_stream, arena_bytes, WS_BATCH_ARENA, &st);
+    if (st) return st;
+    b->arena_ptr = b->arena_lease.ptr();
   } else if (arena_bytes) {
     b->arena = std::make_shared<DevBuf>();
     b->arena->device = device;
@@ -154,8 +165,9 @@ int batch_build(int device, const zkgpu_job* cjobs, uint32_t count, Batch** out,
   memcpy(b->host_meta.data(), all.data(), desc_bytes);
   memcpy(b->host_meta.data() + ts_off, starts.data(), ts_bytes);
   if (opt.use_ws) {
-    b->meta_ptr = e->workspace(opt.ws_stream, meta_bytes, WS_BATCH_META);
-    if (!b->meta_ptr) return ZK_ERR_NOMEM;
+    b->meta_lease = e->lease_workspace(opt.ws_stream, meta_bytes, WS_BATCH_META, &st);
+    if (st) return st;
+    b->meta_ptr = b->meta_lease.ptr();
     if (hipMemcpyAsync(b->meta_ptr, b->host_meta.data(), meta_bytes, hipMemcpyHostToDevice, opt.ws_stream) !=
         hipSuccess)
       return ZK_ERR_HIP;
@@ -238,7 +250,17 @@ int zkgpu_batch_launch(uint64_t handle, void* stream) {
   if (!b) return ZK_ERR_INVALID;
   int st = ZK_OK;
   if (!Engine::get(b->device, &st)) return st;
-  return hipGraphLaunch(b->exec, (hipStream_t)stream) == hipSuccess ? ZK_OK : ZK_ERR_HIP;
+  hipStream_t s = (hipStream_t)stream;
+  std::lock_guard<std::mutex> g(b->launch_mu);
+  hipEvent_t ev = nullptr;
+  for (auto& kv : b->launched)
+    if (kv.first == s) ev = kv.second;
+  if (!ev) {
+    if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) return ZK_ERR_HIP;
+    b->launched.emplace_back(s, ev);
+  }
+  if (hipGraphLaunch(b->exec, s) != hipSuccess) return ZK_ERR_HIP;
+  return hipEventRecord(ev, s) == hipSuccess ? ZK_OK : ZK_ERR_HIP;
 }
 
 int zkgpu_batch_info(uint64_t handle, uint32_t* levels, uint32_t* tiles, uint64_t* elements) {
@@ -258,7 +280,7 @@ int zkgpu_batch_destroy(uint64_t handle) {
   Batch* b = (Batch*)(uintptr_t)handle;
   if (!b) return ZK_ERR_INVALID;
   int st = ZK_OK;
-  if (Engine::get(b->device, &st)) (void)hipDeviceSynchronize();  // queued launches may still read its buffers
+  if (Engine::get(b->device, &st)) b->wait_launches();  // queued launches may still read its buffers
   delete b;
   return ZK_OK;
 }
@@ -316,20 +338,25 @@ uint64_t zkgpu_coeff_to_extended_parts_api(size_t device_id, const uint64_t* coe
   int st = ZK_OK;
   Engine* e = Engine::get((int)device_id, &st);
   if (!e) return (uint64_t)st;
-  hipStream_t s = e->thread_stream();
-  if (!s) return ZK_ERR_HIP;
   const size_t bytes = (size_t)32 << log_n;
-  void* d_in = e->workspace(s, bytes, WS_PARTS_IN);
-  void* d_out = e->workspace(s, bytes * count, WS_PARTS_OUT);
-  if (!d_in || !d_out) return ZK_ERR_NOMEM;
+  HostLease ctx(e, bytes, 0, bytes * count);
+  if (ctx.status()) return (uint64_t)ctx.status();
+  hipStream_t s = ctx->stream;
+  void* d_in = ctx->io->ptr;
+  void* d_out = ctx->aux->ptr;
   std::vector<void*> outs(count);
   for (uint64_t j = 0; j < count; ++j) outs[j] = (char*)d_out + j * bytes;
   st = stage_h2d((int)device_id, s, d_in, coeffs, bytes);
   if (st == ZK_OK)
     st = zkgpu_coeff_to_extended_parts_device((int)device_id, d_in, outs.data(), (uint32_t)log_n, omega, cosets,
                                               (uint32_t)count, s);
-  if (st) return (uint64_t)st;
-  return (uint64_t)stage_d2h((int)device_id, s, parts, d_out, bytes * count);
+  if (st) {
+    (void)hipStreamSynchronize(s);
+    return (uint64_t)st;
+  }
+  st = stage_d2h((int)device_id, s, parts, d_out, bytes * count);
+  if (st) (void)hipStreamSynchronize(s);
+  return (uint64_t)st;
 }
 
 int zkgpu_ntt_batch_device(int device_id, const zkgpu_job* jobs, uint32_t count, void* stream) {

@@ -1,6 +1,9 @@
 // extern "C" entry points of libzkgpu_ntt.so (declared in include/zkgpu_ntt.h).
 #include <hip/hip_runtime.h>
+#include <stdlib.h>
 #include <string.h>
+
+#include <atomic>
 
 #include "../../include/zkgpu_ntt.h"
 #include "engine.hpp"
@@ -13,9 +16,33 @@ bool size_ok(uint64_t n, uint64_t log_n) {
   return log_n >= 1 && log_n <= MAX_LOG_N && n == (1ull << log_n);
 }
 
-// One synchronous host-buffer transform: upload, transform in place on the
-// calling thread's stream, download. The reference does the same per call
-// with fresh allocations and device-wide syncs (cuda/fft.cu:275-405).
+std::atomic<int> g_validate{-1};  // -1: not read yet
+bool validation_on() {
+  int v = g_validate.load(std::memory_order_relaxed);
+  if (v < 0) {
+    const char* e = getenv("ZKGPU_VALIDATE");
+    v = (e && *e && *e != '0') ? 1 : 0;
+    int expect = -1;
+    g_validate.compare_exchange_strong(expect, v);
+    v = g_validate.load();
+  }
+  return v == 1;
+}
+
+// With validation on: ZK_ERR_NONCANONICAL when some of the count elements at
+// d (device) are >= r. Synchronises s.
+int validate_device(void* d, uint64_t count, uint32_t* d_flag, hipStream_t s) {
+  uint32_t flag = 0;
+  if (launch_reset_counters(d_flag, 1, s) != hipSuccess) return ZK_ERR_HIP;
+  if (launch_check_canonical(d, count, d_flag, s) != hipSuccess) return ZK_ERR_HIP;
+  if (hipMemcpyAsync(&flag, d_flag, sizeof(uint32_t), hipMemcpyDeviceToHost, s) != hipSuccess) return ZK_ERR_HIP;
+  if (hipStreamSynchronize(s) != hipSuccess) return ZK_ERR_HIP;
+  return flag ? ZK_ERR_NONCANONICAL : ZK_OK;
+}
+
+// One synchronous host-buffer transform on a host-call context: upload,
+// transform in place, download. The reference does the same per call with
+// fresh allocations and device-wide syncs (cuda/fft.cu:275-405).
 int host_call(size_t device_id, uint64_t* req, uint64_t log_n, int op, const uint64_t* omega,
               const uint64_t* divisor, const uint64_t* c) {
   if (!req || !omega) return ZK_ERR_INVALID;
@@ -24,11 +51,13 @@ int host_call(size_t device_id, uint64_t* req, uint64_t log_n, int op, const uin
   int st = ZK_OK;
   Engine* e = Engine::get((int)device_id, &st);
   if (!e) return st;
-  hipStream_t s = e->thread_stream();
-  if (!s) return ZK_ERR_HIP;
   const size_t bytes = (size_t)32 << log_n;
-  void* d = e->workspace(s, bytes, WS_HOST_IO);
-  if (!d) return ZK_ERR_NOMEM;
+  const bool multi = plan_for((uint32_t)log_n, PLAN_SINGLE).npass > 1;
+  const bool check = validation_on();
+  HostLease ctx(e, bytes, multi ? (size_t)36 << log_n : 0, check ? 256 : 0);
+  if (ctx.status()) return ctx.status();
+  hipStream_t s = ctx->stream;
+  void* d = ctx->io->ptr;
   JobSpec job;
   job.in = d;
   job.out = d;
@@ -38,12 +67,15 @@ int host_call(size_t device_id, uint64_t* req, uint64_t log_n, int op, const uin
   if (divisor) job.divisor = host::from_ptr(divisor);
   if (c) job.coset = host::from_ptr(c);
   st = stage_h2d((int)device_id, s, d, req, bytes);
-  if (st == ZK_OK) st = e->run(job, s);
+  if (st == ZK_OK && check) st = validate_device(d, 1ull << log_n, (uint32_t*)ctx->aux->ptr, s);
+  if (st == ZK_OK) st = e->run(job, s, multi ? ctx->planar->ptr : nullptr);
   if (st) {
-    (void)hipStreamSynchronize(s);
+    (void)hipStreamSynchronize(s);  // the context goes back with an idle stream
     return st;
   }
-  return stage_d2h((int)device_id, s, req, d, bytes);
+  st = stage_d2h((int)device_id, s, req, d, bytes);
+  if (st) (void)hipStreamSynchronize(s);
+  return st;
 }
 
 // Standard-domain constants (omega, omega^-1, n^-1, g, g^-1) for every
@@ -76,19 +108,21 @@ uint64_t do_fft_ffi(size_t device_id) {
   int st = ZK_OK;
   Engine* e = Engine::get((int)device_id, &st);
   if (!e) return (uint64_t)st;
-  hipStream_t s = e->thread_stream();
   const uint32_t k = 10;
   const size_t bytes = (size_t)32 << k;
-  void* a = e->workspace(s, 2 * bytes, WS_SMOKE);
-  if (!a) return ZK_ERR_NOMEM;
+  HostLease ctx(e, 2 * bytes, 0, 0);
+  if (ctx.status()) return (uint64_t)ctx.status();
+  hipStream_t s = ctx->stream;
+  void* a = ctx->io->ptr;
   void* b = (char*)a + bytes;
-  if (launch_fill_random(a, 1, 0, 1ull << k, s) != hipSuccess) return ZK_ERR_HIP;
-  if (zkgpu_ntt_std_device((int)device_id, a, b, k, ZKGPU_OP_FWD, s)) return ZK_ERR_HIP;
-  if (zkgpu_ntt_std_device((int)device_id, b, b, k, ZKGPU_OP_INV, s)) return ZK_ERR_HIP;
   std::vector<uint64_t> ha(4u << k), hb(4u << k);
-  if (hipMemcpyAsync(ha.data(), a, bytes, hipMemcpyDeviceToHost, s) != hipSuccess) return ZK_ERR_HIP;
-  if (hipMemcpyAsync(hb.data(), b, bytes, hipMemcpyDeviceToHost, s) != hipSuccess) return ZK_ERR_HIP;
-  if (hipStreamSynchronize(s) != hipSuccess) return ZK_ERR_HIP;
+  st = launch_fill_random(a, 1, 0, 1ull << k, s) == hipSuccess ? ZK_OK : ZK_ERR_HIP;
+  if (st == ZK_OK) st = zkgpu_ntt_std_device((int)device_id, a, b, k, ZKGPU_OP_FWD, s);
+  if (st == ZK_OK) st = zkgpu_ntt_std_device((int)device_id, b, b, k, ZKGPU_OP_INV, s);
+  if (st == ZK_OK && hipMemcpyAsync(ha.data(), a, bytes, hipMemcpyDeviceToHost, s) != hipSuccess) st = ZK_ERR_HIP;
+  if (st == ZK_OK && hipMemcpyAsync(hb.data(), b, bytes, hipMemcpyDeviceToHost, s) != hipSuccess) st = ZK_ERR_HIP;
+  if (hipStreamSynchronize(s) != hipSuccess && st == ZK_OK) st = ZK_ERR_HIP;
+  if (st) return (uint64_t)st;
   return ha == hb ? ZK_OK : ZK_ERR_HIP;
 }
 
@@ -180,10 +214,7 @@ int zkgpu_prepare(int device_id, uint32_t log_n, int op, const uint64_t* omega, 
   if (divisor) job.divisor = host::from_ptr(divisor);
   if (c) job.coset = host::from_ptr(c);
   JobTables t;
-  hipStream_t s = e->thread_stream();
-  st = e->prepare(job, t, s);
-  if (st) return st;
-  return hipStreamSynchronize(s) == hipSuccess ? ZK_OK : ZK_ERR_HIP;
+  return e->prepare(job, t, nullptr);  // built on the engine's build stream and synchronised there
 }
 
 int zkgpu_fill_random(int device_id, void* d_out, uint64_t count, uint64_t seed, uint64_t start, void* stream) {
@@ -307,6 +338,55 @@ int zkgpu_probe_mad_rate(int device_id, double* mads_per_s) {
   return ZK_OK;
 }
 
-int zkgpu_version(void) { return 100; }
+int zkgpu_set_validation(int enable) {
+  const int prev = validation_on() ? 1 : 0;
+  g_validate.store(enable ? 1 : 0);
+  return prev;
+}
+
+int zkgpu_debug_fail_allocs(uint32_t skip, uint32_t count) {
+  Engine::debug_fail_allocs(skip, count);
+  return ZK_OK;
+}
+
+uint64_t zkgpu_host_held_bytes(int device_id, uint64_t* peak_bytes) {
+  int st = ZK_OK;
+  Engine* e = Engine::get(device_id, &st);
+  if (peak_bytes) *peak_bytes = e ? (uint64_t)e->host_peak_bytes() : 0;
+  return e ? (uint64_t)e->host_held_bytes() : 0;
+}
+
+uint64_t zkgpu_set_host_budget(int device_id, uint64_t bytes) {
+  int st = ZK_OK;
+  Engine* e = Engine::get(device_id, &st);
+  return e ? (uint64_t)e->set_host_budget((size_t)bytes) : 0;
+}
+
+int zkgpu_pass_timing(int device_id, int enable) {
+  int st = ZK_OK;
+  Engine* e = Engine::get(device_id, &st);
+  if (!e) return st;
+  e->set_pass_timing(enable != 0);
+  return ZK_OK;
+}
+
+int zkgpu_pass_times(int device_id, zkgpu_kernel_time* out, uint32_t max_out, uint32_t* count) {
+  if (!count || (max_out && !out)) return ZK_ERR_INVALID;
+  int st = ZK_OK;
+  Engine* e = Engine::get(device_id, &st);
+  if (!e) return st;
+  std::vector<Engine::KernelTime> kt;
+  st = e->pass_times(kt);
+  *count = (uint32_t)kt.size();
+  for (uint32_t i = 0; i < kt.size() && i < max_out; ++i) {
+    memset(&out[i], 0, sizeof(out[i]));
+    strncpy(out[i].name, kt[i].name.c_str(), sizeof(out[i].name) - 1);
+    out[i].launches = kt[i].launches;
+    out[i].total_ms = kt[i].total_ms;
+  }
+  return st;
+}
+
+int zkgpu_version(void) { return 200; }
 
 }  // extern "C"

@@ -9,6 +9,8 @@
 #include <string.h>
 
 #include <algorithm>
+#include <array>
+#include <atomic>
 
 namespace zkgpu {
 
@@ -26,6 +28,10 @@ bool plan_valid(const Plan& p, uint32_t max_tile_log) {
   for (uint32_t i = 0; i < p.npass; ++i) {
     if (p.digit[i] < 1 || p.digit[i] > max_tile_log) return false;
     if (p.digit[i] + p.log_c[i] > max_tile_log) return false;
+    // a tile's threads (8 elements each, fewer for 1- and 2-bit digits) must
+    // fit one workgroup of the kernel that runs it (512 threads for 2^12
+    // tiles, 256 in batches)
+    if (tile_threads(p.digit[i], p.log_c[i], 3) > (1u << (max_tile_log - 3))) return false;
     sum += p.digit[i];
   }
   if (sum != p.log_n) return false;
@@ -200,6 +206,17 @@ Engine::Engine(int device) : device_(device) {
   size_t mb = 32768;  // resident table budget
   if (const char* e = getenv("ZKGPU_TABLE_CACHE_MB")) mb = (size_t)strtoull(e, nullptr, 10);
   table_cap_ = mb << 20;
+  size_t ws_mb = 65536;  // device-API workspaces: idle ones are freed above this
+  if (const char* e = getenv("ZKGPU_WORKSPACE_CAP_MB")) ws_mb = (size_t)strtoull(e, nullptr, 10);
+  ws_cap_ = ws_mb << 20;
+  size_t host_mb = 32768;  // HBM budget of the host-buffer entry points
+  if (const char* e = getenv("ZKGPU_HOST_BUDGET_MB")) host_mb = (size_t)strtoull(e, nullptr, 10);
+  host_budget_ = host_budget_default_ = host_mb << 20;
+  unsigned nctx = 4;  // host-call contexts (streams) shared by every caller thread
+  if (const char* e = getenv("ZKGPU_HOST_STREAMS")) nctx = (unsigned)strtoul(e, nullptr, 10);
+  nctx = std::max(1u, std::min(nctx, 64u));
+  for (unsigned i = 0; i < nctx; ++i) host_ctx_.emplace_back(new HostCtx());
+  (void)hipStreamCreateWithFlags(&build_stream_, hipStreamNonBlocking);
 }
 
 std::string Engine::last_error() {
@@ -211,7 +228,28 @@ void Engine::set_error(const std::string& e) {
   last_error_ = e;
 }
 
+static std::atomic<uint32_t> g_fail_skip{0}, g_fail_count{0};
+void Engine::debug_fail_allocs(uint32_t skip, uint32_t count) {
+  g_fail_count.store(0);
+  g_fail_skip.store(skip);
+  g_fail_count.store(count);
+}
+// true when the test hook asks this allocation to fail
+static bool injected_alloc_failure() {
+  if (g_fail_count.load(std::memory_order_relaxed) == 0) return false;
+  uint32_t sk = g_fail_skip.load();
+  while (sk > 0) {
+    if (g_fail_skip.compare_exchange_weak(sk, sk - 1)) return false;
+  }
+  uint32_t c = g_fail_count.load();
+  while (c > 0) {
+    if (g_fail_count.compare_exchange_weak(c, c - 1)) return true;
+  }
+  return false;
+}
+
 std::shared_ptr<DevBuf> Engine::alloc(size_t bytes) {
+  if (injected_alloc_failure()) return nullptr;
   auto b = std::make_shared<DevBuf>();
   b->device = device_;
   b->bytes = bytes;
@@ -223,64 +261,297 @@ std::shared_ptr<DevBuf> Engine::alloc(size_t bytes) {
   return b;
 }
 
-hipStream_t Engine::thread_stream() {
-  thread_local std::map<int, hipStream_t> streams;
-  auto it = streams.find(device_);
-  if (it != streams.end()) return it->second;
-  hipStream_t s = nullptr;
-  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return nullptr;
-  streams[device_] = s;
-  return s;
+Marker* Engine::marker(hipStream_t s) {
+  // mu_ held
+  auto& m = markers_[s];
+  if (!m) {
+    m.reset(new Marker());
+    m->stream = s;
+    // completion only: nothing reads device memory through the marker, so no
+    // system-scope release (an L2 write-back) is attached to the dispatch
+    if (hipEventCreateWithFlags(&m->ev, hipEventDisableTiming | hipEventDisableSystemFence) != hipSuccess) {
+      (void)hipGetLastError();
+      m->ev = nullptr;
+    }
+  }
+  return m.get();
 }
 
-void* Engine::workspace(hipStream_t s, size_t bytes, int slot) {
+Marker* Engine::stream_marker(hipStream_t s) {
   std::lock_guard<std::mutex> g(mu_);
-  auto key = std::make_pair(s, slot);
-  auto it = ws_.find(key);
-  if (it != ws_.end() && it->second->bytes >= bytes) return it->second->ptr;
-  if (it != ws_.end()) {
-    // the old buffer may still be in use by work queued on s
+  return marker(s);
+}
+
+void Engine::note_users(const JobTables& t, Marker* m) {
+  std::lock_guard<std::mutex> g(mu_);
+  auto add = [&](const std::shared_ptr<DevBuf>& b) {
+    if (b && std::find(b->users.begin(), b->users.end(), m) == b->users.end()) b->users.push_back(m);
+  };
+  add(t.roots);
+  for (const auto& w : t.w) add(w);
+  add(t.pre);
+  add(t.post);
+}
+
+void Engine::set_pass_timing(bool on) {
+  timing_.store(on);
+}
+
+int Engine::pass_times(std::vector<KernelTime>& out) {
+  out.clear();
+  std::lock_guard<std::mutex> g(timing_mu_);
+  int st = ZK_OK;
+  for (auto& tp : timed_) {
+    float ms = 0;
+    if (hipEventSynchronize(tp.b) != hipSuccess || hipEventElapsedTime(&ms, tp.a, tp.b) != hipSuccess) {
+      (void)hipGetLastError();
+      st = ZK_ERR_HIP;
+    }
+    auto it = std::find_if(out.begin(), out.end(), [&](const KernelTime& k) { return k.name == tp.name; });
+    if (it == out.end()) {
+      out.push_back(KernelTime());
+      it = out.end() - 1;
+      it->name = tp.name;
+    }
+    ++it->launches;
+    it->total_ms += ms;
+    timing_pool_.push_back(tp.a);
+    timing_pool_.push_back(tp.b);
+  }
+  timed_.clear();
+  return st;
+}
+
+WsLease Engine::lease_workspace(hipStream_t s, size_t bytes, int slot, int* status) {
+  WsLease l;
+  *status = ZK_OK;
+  Workspace* w = nullptr;
+  {
+    std::lock_guard<std::mutex> g(mu_);
+    auto& p = ws_[std::make_pair(s, slot)];
+    if (!p) {
+      p.reset(new Workspace());
+      p->stream = s;
+    }
+    w = p.get();
+  }
+  l.lk_ = std::unique_lock<std::mutex>(w->mu);
+  if (w->buf && w->buf->bytes >= bytes) {
+    l.buf_ = w->buf;
+    return l;
+  }
+  if (w->buf) {
+    // the old buffer may still be read by work queued on s (only on s: the
+    // lease is held while such work is enqueued)
     (void)hipStreamSynchronize(s);
-    ws_.erase(it);
+    std::lock_guard<std::mutex> g(mu_);
+    ws_bytes_ -= w->buf->bytes;
+    w->buf.reset();
+  }
+  {
+    // over the cap: free workspaces of other streams that are neither leased
+    // nor still in use by their stream's queued work
+    std::lock_guard<std::mutex> g(mu_);
+    for (auto it = ws_.begin(); ws_bytes_ + bytes > ws_cap_ && it != ws_.end(); ++it) {
+      Workspace* o = it->second.get();
+      if (o == w || !o->buf) continue;
+      std::unique_lock<std::mutex> ol(o->mu, std::try_to_lock);
+      if (!ol.owns_lock()) continue;
+      auto mk = markers_.find(o->stream);
+      const bool idle = mk == markers_.end() || !mk->second->ev || hipEventQuery(mk->second->ev) == hipSuccess;
+      if (!idle) continue;
+      ws_bytes_ -= o->buf->bytes;
+      o->buf.reset();
+    }
   }
   auto b = alloc(bytes);
-  if (!b) return nullptr;
-  ws_[key] = b;
-  return b->ptr;
+  if (!b) {
+    l.lk_.unlock();
+    *status = ZK_ERR_NOMEM;
+    return WsLease();
+  }
+  {
+    std::lock_guard<std::mutex> g(mu_);
+    ws_bytes_ += bytes;
+  }
+  w->buf = b;
+  l.buf_ = b;
+  return l;
 }
 
-std::shared_ptr<DevBuf> Engine::table(const TableKey& key, size_t bytes, bool* fresh) {
-  std::lock_guard<std::mutex> g(mu_);
-  auto it = tables_.find(key);
-  if (it != tables_.end()) {
-    lru_.remove(key);
-    lru_.push_front(key);
-    *fresh = false;
-    return it->second;
-  }
-  // evict least recently used tables nobody else holds
-  while (table_bytes_ + bytes > table_cap_ && !lru_.empty()) {
-    bool evicted = false;
-    for (auto r = lru_.rbegin(); r != lru_.rend(); ++r) {
-      auto t = tables_.find(*r);
-      if (t != tables_.end() && t->second.use_count() == 1) {
-        (void)hipDeviceSynchronize();  // work on other streams may still read it
-        table_bytes_ -= t->second->bytes;
-        tables_.erase(t);
-        lru_.erase(std::next(r).base());
-        evicted = true;
+int Engine::acquire_host(size_t io_bytes, size_t planar_bytes, size_t aux_bytes, HostCtx** out) {
+  *out = nullptr;
+  const size_t need[3] = {io_bytes, planar_bytes, aux_bytes};
+  auto have = [](const std::shared_ptr<DevBuf>& b) { return b ? b->bytes : (size_t)0; };
+  auto slots = [](HostCtx* h) {
+    std::array<std::shared_ptr<DevBuf>*, 3> a = {&h->io, &h->planar, &h->aux};
+    return a;
+  };
+  std::unique_lock<std::mutex> lk(host_mu_);
+  HostCtx* c = nullptr;
+  bool grows[3] = {false, false, false};
+  for (;;) {
+    c = nullptr;
+    size_t best = SIZE_MAX;
+    bool any_busy = false;
+    for (auto& h : host_ctx_) {
+      if (h->busy) {
+        any_busy = true;
+        continue;
+      }
+      // prefer the idle context whose buffers need the least growth
+      size_t g = 0;
+      auto sl = slots(h.get());
+      for (int i = 0; i < 3; ++i) g += need[i] > have(*sl[i]) ? need[i] : 0;
+      if (g < best) {
+        best = g;
+        c = h.get();
+      }
+    }
+    if (c) {
+      // buffers that grow are replaced: the old ones stop counting
+      auto sl = slots(c);
+      size_t freed = 0;
+      for (int i = 0; i < 3; ++i)
+        if (need[i] > have(*sl[i])) freed += have(*sl[i]);
+      if (host_held_ - freed + best > host_budget_) {
+        // release the buffers of the other idle contexts (their streams are idle)
+        for (auto& h : host_ctx_) {
+          if (h->busy || h.get() == c) continue;
+          host_held_ -= h->held();
+          h->io.reset();
+          h->planar.reset();
+          h->aux.reset();
+        }
+      }
+      // over budget with other calls running: wait for them (a call larger
+      // than the whole budget still runs, alone)
+      if (host_held_ - freed + best <= host_budget_ || !any_busy) {
+        // reserve the growth now, so that concurrent acquirers see it
+        for (int i = 0; i < 3; ++i) {
+          grows[i] = need[i] > have(*sl[i]);
+          if (!grows[i]) continue;
+          host_held_ -= have(*sl[i]);
+          sl[i]->reset();  // the context's stream is idle
+          host_held_ += need[i];
+        }
+        host_peak_ = std::max(host_peak_, host_held_);
         break;
       }
     }
-    if (!evicted) break;
+    host_cv_.wait(lk);
   }
-  auto b = alloc(bytes);
-  if (!b) return nullptr;
-  tables_[key] = b;
+  c->busy = true;
+  lk.unlock();
+  int st = ZK_OK;
+  if (!c->stream && hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    (void)hipGetLastError();
+    c->stream = nullptr;
+    st = ZK_ERR_HIP;
+  }
+  auto sl = slots(c);
+  size_t unallocated = 0;
+  for (int i = 0; i < 3; ++i) {
+    if (!grows[i]) continue;
+    if (st == ZK_OK) *sl[i] = alloc(need[i]);
+    if (!*sl[i]) {
+      unallocated += need[i];
+      if (st == ZK_OK) st = ZK_ERR_NOMEM;
+    }
+  }
+  if (st) {
+    {
+      std::lock_guard<std::mutex> g(host_mu_);
+      host_held_ -= unallocated;
+    }
+    release_host(c);
+    return st;
+  }
+  *out = c;
+  return ZK_OK;
+}
+
+void Engine::release_host(HostCtx* c) {
+  {
+    std::lock_guard<std::mutex> g(host_mu_);
+    c->busy = false;
+  }
+  host_cv_.notify_all();
+}
+
+size_t Engine::host_held_bytes() {
+  std::lock_guard<std::mutex> g(host_mu_);
+  return host_held_;
+}
+size_t Engine::host_peak_bytes() {
+  std::lock_guard<std::mutex> g(host_mu_);
+  return host_peak_;
+}
+size_t Engine::set_host_budget(size_t bytes) {
+  std::lock_guard<std::mutex> g(host_mu_);
+  const size_t prev = host_budget_;
+  host_budget_ = bytes ? bytes : host_budget_default_;
+  for (auto& h : host_ctx_) {
+    if (host_held_ <= host_budget_) break;
+    if (h->busy) continue;
+    host_held_ -= h->held();
+    h->io.reset();
+    h->planar.reset();
+    h->aux.reset();
+  }
+  host_peak_ = host_held_;
+  return prev;
+}
+
+std::shared_ptr<DevBuf> Engine::cached_table(const TableKey& key) {
+  std::lock_guard<std::mutex> g(mu_);
+  auto it = tables_.find(key);
+  if (it == tables_.end()) return nullptr;
+  lru_.remove(key);
   lru_.push_front(key);
-  table_bytes_ += bytes;
-  *fresh = true;
-  return b;
+  return it->second;
+}
+
+// A new (unpublished) table buffer, after evicting least recently used tables
+// that no job holds. An evicted table is freed once every stream that ran
+// work reading it has passed that work (its users' markers).
+std::shared_ptr<DevBuf> Engine::new_table(size_t bytes) {
+  std::vector<std::shared_ptr<DevBuf>> victims;
+  {
+    std::lock_guard<std::mutex> g(mu_);
+    while (table_bytes_ + bytes > table_cap_ && !lru_.empty()) {
+      bool evicted = false;
+      for (auto r = lru_.rbegin(); r != lru_.rend(); ++r) {
+        auto t = tables_.find(*r);
+        if (t != tables_.end() && t->second.use_count() == 1) {
+          table_bytes_ -= t->second->bytes;
+          victims.push_back(t->second);
+          tables_.erase(t);
+          lru_.erase(std::next(r).base());
+          evicted = true;
+          break;
+        }
+      }
+      if (!evicted) break;
+    }
+  }
+  for (auto& v : victims) {
+    for (Marker* m : v->users)
+      if (m->ev) (void)hipEventSynchronize(m->ev);
+  }
+  victims.clear();
+  return alloc(bytes);
+}
+
+void Engine::publish_tables(std::vector<std::pair<TableKey, std::shared_ptr<DevBuf>>>& fresh) {
+  std::lock_guard<std::mutex> g(mu_);
+  for (auto& kv : fresh) {
+    if (tables_.count(kv.first)) continue;
+    tables_[kv.first] = kv.second;
+    lru_.push_front(kv.first);
+    table_bytes_ += kv.second->bytes;
+  }
+  fresh.clear();
 }
 
 size_t Engine::table_bytes() {
@@ -288,11 +559,22 @@ size_t Engine::table_bytes() {
   return table_bytes_;
 }
 void Engine::drop_tables() {
-  (void)hipDeviceSynchronize();
-  std::lock_guard<std::mutex> g(mu_);
-  tables_.clear();
-  lru_.clear();
-  table_bytes_ = 0;
+  std::lock_guard<std::mutex> b(build_mu_);
+  std::vector<std::shared_ptr<DevBuf>> all;
+  {
+    std::lock_guard<std::mutex> g(mu_);
+    for (auto& kv : tables_) all.push_back(kv.second);
+    tables_.clear();
+    lru_.clear();
+    table_bytes_ = 0;
+  }
+  // tables still held by jobs (batches, served jobs) are freed when those
+  // release them; the rest once their users' work has finished
+  for (auto& v : all) {
+    if (v.use_count() > 1) continue;
+    for (Marker* m : v->users)
+      if (m->ev) (void)hipEventSynchronize(m->ev);
+  }
 }
 
 static TableKey make_key(char kind, uint32_t a, uint32_t b, uint32_t c, const host::Fr& x,
@@ -325,35 +607,55 @@ int Engine::build_pow_lohi(const host::Fr& base, uint64_t step, uint32_t log_cou
 
 int Engine::prepare(const JobSpec& job, JobTables& t, hipStream_t s, int plan_kind) {
   std::lock_guard<std::mutex> build_lock(build_mu_);
+  if (!s) s = build_stream_;
   t = JobTables();
   t.plan = plan_for(job.log_n, plan_kind);
   if (!plan_valid(t.plan, plan_tile_log(plan_kind))) return ZK_ERR_INVALID;
   const Plan& p = t.plan;
   const uint32_t k = job.log_n;
   std::vector<std::shared_ptr<DevBuf>> temps;  // kept alive until the build is synchronised
-  bool built = false;
+  // tables built by this call: published only once their build has finished,
+  // so an error on the way leaves no unbuilt table in the cache
+  std::vector<std::pair<TableKey, std::shared_ptr<DevBuf>>> fresh;
+  // on an error after builds were enqueued, let them finish before their
+  // buffers are dropped
+  struct BuildGuard {
+    hipStream_t s;
+    const std::vector<std::pair<TableKey, std::shared_ptr<DevBuf>>>& fresh;
+    ~BuildGuard() {
+      if (!fresh.empty() && hipStreamSynchronize(s) != hipSuccess) (void)hipGetLastError();
+    }
+  } guard{s, fresh};
+  auto fetch = [&](const TableKey& key, size_t bytes, bool* is_new) -> std::shared_ptr<DevBuf> {
+    auto b = cached_table(key);
+    *is_new = !b;
+    if (!b) {
+      b = new_table(bytes);
+      if (b) fresh.emplace_back(key, b);
+    }
+    return b;
+  };
 
   // roots of the tile DFTs: omega_Nt^e, Nt = 2^min(k, 12)
   {
     const uint32_t lnt = std::min(k, TILE_LOG_MAX);
-    bool fresh = false;
-    t.roots = table(make_key('R', k, lnt, 0, job.omega), (size_t)48 << lnt, &fresh);
+    bool is_new = false;
+    t.roots = fetch(make_key('R', k, lnt, 0, job.omega), (size_t)48 << lnt, &is_new);
     if (!t.roots) return ZK_ERR_NOMEM;
-    if (fresh) {
+    if (is_new) {
       uint32_t w[8];
       host::to_words32(w, job.omega);
       if (launch_pow_table(t.roots->ptr, w, 1ull << (k - lnt), 1u << lnt, true, s) != hipSuccess) return ZK_ERR_HIP;
-      built = true;
     }
   }
   // inter-pass twiddles W_p[t * L + i] = omega_m^(t * i)
   uint32_t log_m = k;
   for (uint32_t i = 0; i + 1 < p.npass; ++i) {
     const uint32_t log_L = log_m - p.digit[i];
-    bool fresh = false;
-    t.w[i] = table(make_key('W', k, log_m, log_L, job.omega), (size_t)32 << log_m, &fresh);
+    bool is_new = false;
+    t.w[i] = fetch(make_key('W', k, log_m, log_L, job.omega), (size_t)32 << log_m, &is_new);
     if (!t.w[i]) return ZK_ERR_NOMEM;
-    if (fresh) {
+    if (is_new) {
       const uint32_t h = (log_m + 1) / 2;
       std::shared_ptr<DevBuf> lo, hi;
       int st = build_pow_lohi(job.omega, 1ull << (k - log_m), log_m, h, lo, hi, s);
@@ -361,7 +663,6 @@ int Engine::prepare(const JobSpec& job, JobTables& t, hipStream_t s, int plan_ki
       if (launch_twiddle_table(t.w[i]->ptr, lo->ptr, hi->ptr, h, log_m, log_L, s) != hipSuccess) return ZK_ERR_HIP;
       temps.push_back(lo);
       temps.push_back(hi);
-      built = true;
     }
     log_m = log_L;
   }
@@ -369,10 +670,10 @@ int Engine::prepare(const JobSpec& job, JobTables& t, hipStream_t s, int plan_ki
   if (job.op == OP_COSET_FWD || job.op == OP_COSET_INV) {
     const bool post = job.op == OP_COSET_INV;
     const TableKey key = make_key(post ? 'Q' : 'P', k, 0, 0, job.coset, post ? &job.divisor : nullptr);
-    bool fresh = false;
-    auto tab = table(key, (size_t)32 << k, &fresh);
+    bool is_new = false;
+    auto tab = fetch(key, (size_t)32 << k, &is_new);
     if (!tab) return ZK_ERR_NOMEM;
-    if (fresh) {
+    if (is_new) {
       const uint32_t h = (k + 1) / 2;
       std::shared_ptr<DevBuf> lo, hi;
       int st = build_pow_lohi(job.coset, 1, k, h, lo, hi, s);
@@ -383,7 +684,6 @@ int Engine::prepare(const JobSpec& job, JobTables& t, hipStream_t s, int plan_ki
         return ZK_ERR_HIP;
       temps.push_back(lo);
       temps.push_back(hi);
-      built = true;
     }
     (post ? t.post : t.pre) = tab;
   }
@@ -391,14 +691,29 @@ int Engine::prepare(const JobSpec& job, JobTables& t, hipStream_t s, int plan_ki
     t.uscale = true;
     host::to_limbs261(t.scale261, job.divisor);
   }
-  if (built) {
-    // publish only finished tables: other streams may use them right away
-    if (hipStreamSynchronize(s) != hipSuccess) return ZK_ERR_HIP;
+  if (!fresh.empty()) {
+    if (hipStreamSynchronize(s) != hipSuccess) {
+      (void)hipGetLastError();
+      return ZK_ERR_HIP;
+    }
+    publish_tables(fresh);
   }
   return ZK_OK;
 }
 
-int Engine::build_descs(const JobSpec& job, const JobTables& t, void* ws, PassDesc* descs, uint32_t* ndesc) const {
+TablePtrs table_ptrs(const JobTables& t) {
+  TablePtrs p;
+  p.plan = t.plan;
+  p.roots = t.roots ? t.roots->ptr : nullptr;
+  for (uint32_t i = 0; i < MAX_PASSES; ++i) p.w[i] = t.w[i] ? t.w[i]->ptr : nullptr;
+  p.pre = t.pre ? t.pre->ptr : nullptr;
+  p.post = t.post ? t.post->ptr : nullptr;
+  p.uscale = t.uscale;
+  memcpy(p.scale261, t.scale261, sizeof(p.scale261));
+  return p;
+}
+
+int build_pass_descs(const JobSpec& job, const TablePtrs& t, void* ws, PassDesc* descs, uint32_t* ndesc) {
   const Plan& p = t.plan;
   const uint32_t k = job.log_n;
   const uint32_t lnt = std::min(k, TILE_LOG_MAX);
@@ -410,7 +725,7 @@ int Engine::build_descs(const JobSpec& job, const JobTables& t, void* ws, PassDe
     memset(&d, 0, sizeof(d));
     const uint32_t N = p.digit[i], c = p.log_c[i];
     const bool last = i + 1 == p.npass;
-    d.roots = t.roots->ptr;
+    d.roots = t.roots;
     d.root_log = lnt;
     d.log_n_tile = N;
     d.log_c = c;
@@ -418,13 +733,13 @@ int Engine::build_descs(const JobSpec& job, const JobTables& t, void* ws, PassDe
     // workspace after it; the previous pass's twiddles are applied on load
     if (i == 0) {
       d.in = job.in;
-      d.pre = t.pre ? t.pre->ptr : nullptr;
+      d.pre = t.pre;
       d.pre_mask = n - 1;
     } else {
       d.in = ws;
       d.in_plane = n;
       d.flags |= PASS_IN_PLANAR;
-      d.pre = t.w[i - 1]->ptr;
+      d.pre = t.w[i - 1];
       d.pre_mask = (1u << prev_log_m) - 1u;
     }
     if (!last) {
@@ -463,7 +778,7 @@ int Engine::build_descs(const JobSpec& job, const JobTables& t, void* ws, PassDe
       d.out_ts = 1u << (k - N);
     }
     if (last) {
-      d.post = t.post ? t.post->ptr : nullptr;
+      d.post = t.post;
       d.flags |= PASS_CANON | (t.uscale ? PASS_USCALE : 0u);
       if (t.uscale) memcpy(d.scale, t.scale261, sizeof(t.scale261));
     }
@@ -498,24 +813,55 @@ int job_from_c(const zkgpu_job& j, JobSpec& out, bool out_optional) {
   return ZK_OK;
 }
 
-int Engine::run(const JobSpec& job, hipStream_t s) {
+int Engine::run(const JobSpec& job, hipStream_t s, void* planar_ws) {
   if (job.log_n < 1 || job.log_n > MAX_LOG_N || !job.in || !job.out) return ZK_ERR_INVALID;
   if (job.op < OP_FWD || job.op > OP_COSET_INV) return ZK_ERR_INVALID;
   JobTables t;
   int st = prepare(job, t, s);
   if (st) return st;
-  void* ws = nullptr;
-  if (t.plan.npass > 1) {
-    ws = workspace(s, (size_t)36 << job.log_n, WS_PLANAR);  // planar limb form
-    if (!ws) return ZK_ERR_NOMEM;
+  WsLease lease;  // held until every pass is enqueued
+  void* ws = planar_ws;
+  if (t.plan.npass > 1 && !ws) {
+    lease = lease_workspace(s, (size_t)36 << job.log_n, WS_PLANAR, &st);  // planar limb form
+    if (st) return st;
+    ws = lease.ptr();
   }
   PassDesc descs[MAX_PASSES];
   uint32_t nd = 0;
   st = build_descs(job, t, ws, descs, &nd);
   if (st) return st;
+  // the stream's marker is tied to the last pass's dispatch (no extra packet)
+  Marker* m = stream_marker(s);
+  const bool timing = timing_.load(std::memory_order_relaxed);
   for (uint32_t i = 0; i < nd; ++i) {
-    if (launch_pass(descs[i], s) != hipSuccess) return ZK_ERR_HIP;
+    hipEvent_t a = nullptr, b = nullptr;
+    if (timing) {
+      std::lock_guard<std::mutex> g(timing_mu_);
+      for (hipEvent_t* e : {&a, &b}) {
+        if (!timing_pool_.empty()) {
+          *e = timing_pool_.back();
+          timing_pool_.pop_back();
+        } else if (hipEventCreateWithFlags(e, hipEventDisableSystemFence) != hipSuccess) {
+          (void)hipGetLastError();
+          *e = nullptr;
+        }
+      }
+    } else if (i + 1 == nd) {
+      b = m->ev;
+    }
+    if (launch_pass(descs[i], s, a, b) != hipSuccess) return ZK_ERR_HIP;
+    if (timing && a && b) {
+      std::lock_guard<std::mutex> g(timing_mu_);
+      TimedPass tp;
+      tp.a = a;
+      tp.b = b;
+      tp.name = std::string(pass_kernel_narrow(descs[i]) ? "ntt_pass_kernel_narrow<" : "ntt_pass_kernel<") +
+                std::to_string(descs[i].log_n_tile) + ">";
+      timed_.push_back(tp);
+    }
   }
+  if (timing && m->ev && hipEventRecord(m->ev, s) != hipSuccess) return ZK_ERR_HIP;
+  note_users(t, m);
   return ZK_OK;
 }
 

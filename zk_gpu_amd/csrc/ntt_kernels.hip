@@ -15,6 +15,7 @@
 // cuda/fft.cu:223-232) on the first load; the inter-pass twiddle, the
 // inverse's divisor (reference cuda/fft.cu:153-155) and the coset-inverse
 // post-scale on the last store, together with the final canonicalisation.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -726,30 +727,62 @@ uint32_t persistent_wgs_per_cu() { return (uint32_t)PERSIST_WGS; }
 // planes of the largest batch tile, then the 16-byte item slot
 static uint32_t persistent_lds_bytes() { return 4u * BATCH_ITEM_WORD + 16u; }
 
+// Which pass-kernel family runs a descriptor (see ntt_pass_kernel_narrow).
 template <int LOGN>
-static void launch_one(const PassDesc& d, hipStream_t s) {
-  const uint32_t tl = LOGN + d.log_c;
+static bool use_narrow(const PassDesc& d) {
   if constexpr (LOGN <= (int)NARROW_TILE_LOG) {
+    const uint32_t tl = LOGN + d.log_c;
     const uint32_t threads = tile_threads(LOGN, d.log_c, NarrowXchg::le_max);
     const uint64_t waves = ((uint64_t)d.num_tiles * threads + 63) / 64;
-    if (tl <= NARROW_TILE_LOG && waves > g_narrow_min_waves.load(std::memory_order_relaxed)) {
-      ntt_pass_kernel_narrow<LOGN><<<d.num_tiles, threads, (4u * NarrowXchg::words) << tl, s>>>(d);
-      return;
+    return tl <= NARROW_TILE_LOG && waves > g_narrow_min_waves.load(std::memory_order_relaxed);
+  }
+  return false;
+}
+
+// <<<>>> when no event is asked for; otherwise hipExtLaunchKernel, which ties
+// the events to the dispatch itself (no separate event packet on the stream).
+template <class K>
+static hipError_t launch_kernel(K kernel, uint32_t grid, uint32_t threads, uint32_t lds, hipStream_t s,
+                                const PassDesc& d, hipEvent_t start, hipEvent_t stop) {
+  if (!start && !stop) {
+    kernel<<<grid, threads, lds, s>>>(d);
+    return hipGetLastError();
+  }
+  PassDesc arg = d;
+  void* args[] = {&arg};
+  return hipExtLaunchKernel((const void*)kernel, dim3(grid), dim3(threads), args, lds, s, start, stop, 0);
+}
+
+template <int LOGN>
+static hipError_t launch_one(const PassDesc& d, hipStream_t s, hipEvent_t start, hipEvent_t stop) {
+  const uint32_t tl = LOGN + d.log_c;
+  if constexpr (LOGN <= (int)NARROW_TILE_LOG) {
+    if (use_narrow<LOGN>(d)) {
+      return launch_kernel(ntt_pass_kernel_narrow<LOGN>, d.num_tiles, tile_threads(LOGN, d.log_c, NarrowXchg::le_max),
+                           (4u * NarrowXchg::words) << tl, s, d, start, stop);
     }
   }
-  ntt_pass_kernel<LOGN><<<d.num_tiles, tile_threads(LOGN, d.log_c, PassXchg::le_max),
-                          (4u * PassXchg::words) << tl, s>>>(d);
+  return launch_kernel(ntt_pass_kernel<LOGN>, d.num_tiles, tile_threads(LOGN, d.log_c, PassXchg::le_max),
+                       (4u * PassXchg::words) << tl, s, d, start, stop);
 }
 #define ZK_TILE_CASES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12)
 
-hipError_t launch_pass(const PassDesc& d, hipStream_t s) {
+hipError_t launch_pass(const PassDesc& d, hipStream_t s, hipEvent_t start, hipEvent_t stop) {
   switch (d.log_n_tile) {
-#define X(L) case L: launch_one<L>(d, s); break;
+#define X(L) case L: return launch_one<L>(d, s, start, stop);
     ZK_TILE_CASES(X)
 #undef X
     default: return hipErrorInvalidValue;
   }
-  return hipGetLastError();
+}
+
+bool pass_kernel_narrow(const PassDesc& d) {
+  switch (d.log_n_tile) {
+#define X(L) case L: return use_narrow<L>(d);
+    ZK_TILE_CASES(X)
+#undef X
+    default: return false;
+  }
 }
 
 hipError_t launch_persistent(const PassDesc* d_descs, const uint32_t* d_tile_start, uint32_t ndesc, uint32_t total,

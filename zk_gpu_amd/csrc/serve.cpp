@@ -120,6 +120,7 @@ struct SJob {
   size_t footprint = 0;
   int status = ZK_OK;
   bool done = false;
+  uint32_t iters = 0, iter_jobs_max = 0;  // iterations run, most jobs in one of them
   uint64_t t_submit = 0, t_admit = 0, t_start = 0, t_done = 0;
 };
 
@@ -226,6 +227,8 @@ class Server {
       out->t_admit_ns = j->t_admit;
       out->t_start_ns = j->t_start;
       out->t_done_ns = j->t_done;
+      out->iterations = j->iters;
+      out->iter_jobs_max = j->iter_jobs_max;
     }
     return ZK_OK;
   }
@@ -277,7 +280,7 @@ class Server {
       SJob* j = pending_.front();
       if (in_use_ + j->footprint > budget_) break;  // arrival order: no overtaking
       pending_.pop_front();
-      int st = eng_->prepare(j->spec, j->tables, eng_->thread_stream(), PLAN_BATCH);
+      int st = eng_->prepare(j->spec, j->tables, nullptr, PLAN_BATCH);
       if (st == ZK_OK && j->tables.plan.npass != j->npass) st = ZK_ERR_INVALID;  // plan changed since submit
       if (st == ZK_OK && j->ws_bytes) {
         j->ws = pool_.get(j->ws_bytes);
@@ -330,6 +333,10 @@ class Server {
       if (++j->next == j->npass) it.finishing.push_back(j);
     }
     hcnt[0] = 0;
+    for (SJob* j : order) {
+      ++j->iters;
+      j->iter_jobs_max = std::max(j->iter_jobs_max, nd);
+    }
     // jobs in their last pass leave the admitted list now; the iteration's
     // completion finishes them
     active_.erase(std::remove_if(active_.begin(), active_.end(), [](SJob* j) { return j->next == j->npass; }),
