@@ -87,12 +87,33 @@ OPTIONAL = {}
 _lib = None
 
 
+def _share_torch_hip_runtime():
+    """PyTorch ships its own HIP runtime (torch/lib/libamdhip64.so, soname
+    libamdhip64.so.7). The library links the system one by that soname. When
+    torch is imported first, the library binds to torch's copy; loaded before
+    torch, it would bring the system copy and torch a second one, and the
+    library's runtime then sees no device. Loading torch's runtime file first
+    (without importing torch) makes every later load, ours and torch's, use
+    that one copy."""
+    import importlib.util
+    try:
+        spec = importlib.util.find_spec("torch")
+    except (ImportError, ValueError):
+        return
+    if spec is None or not spec.origin:
+        return
+    path = os.path.join(os.path.dirname(spec.origin), "lib", "libamdhip64.so")
+    if os.path.exists(path):
+        ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+
+
 def lib():
     """The loaded library (raises when it has not been built)."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m zk_gpu_amd.build`")
+        _share_torch_hip_runtime()
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in {**SIGNATURES, **OPTIONAL}.items():
             if not hasattr(L, name):
