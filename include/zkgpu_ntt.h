@@ -31,6 +31,7 @@ extern "C" {
 #define ZKGPU_ERR_NONCANONICAL 5 /* an input element is >= r (host-buffer entry points, validation on) */
 #define ZKGPU_ERR_BUDGET 6       /* a job exceeds the serving HBM budget */
 #define ZKGPU_ERR_SHUTDOWN 7     /* the server is stopping */
+#define ZKGPU_ERR_SCRATCH 8      /* size beyond the no-allocation entry point's scratch limit */
 
 #define ZKGPU_OP_FWD 0       /* X_k = sum_i a_i w^(ik) */
 #define ZKGPU_OP_INV 1       /* X_k = divisor * sum_i a_i w^(ik)  (w = omega^-1 for halo2's ifft) */
@@ -89,6 +90,24 @@ uint64_t zkgpu_coset_ifft_api(size_t device_id, uint64_t* req, const uint64_t* o
  * one element each, read before the call returns). */
 int zkgpu_ntt_device(int device_id, const void* d_in, void* d_out, uint32_t log_n, int op,
                      const uint64_t* omega, const uint64_t* divisor, const uint64_t* c, void* stream);
+
+/* zkgpu_ntt_device without any allocation. No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (work under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Same
+ * arguments, kernels and results as zkgpu_ntt_device, but the inter-pass
+ * workspace and the twiddle/coset tables live in a fixed scratch array of
+ * the library (zkgpu_noalloc_scratch_bytes, 120 MiB), held by each GPU the
+ * library runs on until the process ends; no hipMalloc, hipFree or other
+ * allocating call is made. log_n <= 20 (*max_log_n of
+ * zkgpu_noalloc_scratch_bytes); larger sizes return ZKGPU_ERR_SCRATCH. The
+ * scratch keeps the last call's tables, reused by the next call with the same
+ * size, op and constants. All calls on one device share its scratch: they
+ * must be ordered (one stream, or synchronised); calls that overlap race. */
+int zkgpu_ntt_device_noalloc(int device_id, const void* d_in, void* d_out, uint32_t log_n, int op,
+                             const uint64_t* omega, const uint64_t* divisor, const uint64_t* c, void* stream);
+/* Bytes of that scratch array per device; *max_log_n (optional) gets the
+ * largest log_n the no-allocation entry point accepts. */
+uint64_t zkgpu_noalloc_scratch_bytes(uint32_t* max_log_n);
 
 /* Device-resident transform with halo2's standard domain of size 2^log_n:
  * omega = 7^((r-1)/2^28) squared down to order 2^log_n, divisor = n^-1 and

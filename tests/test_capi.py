@@ -42,7 +42,7 @@ def test_drop_in_symbols_match_reference_ffi():
 
 def test_version_and_error_strings():
     assert Z.version() >= 100
-    for code in range(8):
+    for code in range(9):
         s = Z.lib().zkgpu_error_string(code)
         assert s and s != b"unknown error"
     assert Z.lib().zkgpu_error_string(99) == b"unknown error"

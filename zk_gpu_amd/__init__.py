@@ -26,6 +26,7 @@ ERR_NOMEM = 4
 ERR_NONCANONICAL = 5
 ERR_BUDGET = 6
 ERR_SHUTDOWN = 7
+ERR_SCRATCH = 8
 
 OP_FWD = 0
 OP_INV = 1
@@ -49,6 +50,8 @@ SIGNATURES = {
     "zkgpu_coset_ifft_api": (_U64, [_SZ, _P, _P, _U64, _U64, _P, _P]),
     "zkgpu_ntt_device": (_I, [_I, _P, _P, _U32, _I, _P, _P, _P, _P]),
     "zkgpu_ntt_std_device": (_I, [_I, _P, _P, _U32, _I, _P]),
+    "zkgpu_ntt_device_noalloc": (_I, [_I, _P, _P, _U32, _I, _P, _P, _P, _P]),
+    "zkgpu_noalloc_scratch_bytes": (_U64, [_P]),
     "zkgpu_domain_constant": (_I, [_U32, _I, _P]),
     "zkgpu_prepare": (_I, [_I, _U32, _I, _P, _P, _P]),
     "zkgpu_fill_random": (_I, [_I, _P, _U64, _U64, _U64, _P]),

@@ -296,6 +296,7 @@ const char* zkgpu_error_string(int code) {
     case ZK_ERR_NONCANONICAL: return "input element not below r";
     case ZK_ERR_BUDGET: return "job exceeds the HBM budget";
     case ZK_ERR_SHUTDOWN: return "server stopping";
+    case ZK_ERR_SCRATCH: return "size exceeds the no-allocation scratch limit";
     default: return "unknown error";
   }
 }

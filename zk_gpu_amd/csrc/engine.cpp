@@ -186,6 +186,7 @@ Engine* Engine::get(int device, int* status) {
         if (status) *status = ZK_ERR_HIP;
         return nullptr;
       }
+      noalloc_init_device(device);
       it = g_engines.emplace(device, std::unique_ptr<Engine>(new Engine(device))).first;
     }
     e = it->second.get();

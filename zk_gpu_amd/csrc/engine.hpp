@@ -341,5 +341,8 @@ uint64_t mad_probe_ops_per_thread();
 uint32_t tile_threads(uint32_t log_n_tile, uint32_t log_c, uint32_t le_max);  // threads of a tile
 uint32_t batch_items(const PassDesc& d);  // queue items of a persistent-kernel descriptor
 uint32_t persistent_wgs_per_cu();          // resident persistent-kernel workgroups per CU
+// noalloc.hip: once per device, with it current, look up the device address
+// of the no-allocation entry point's scratch array
+void noalloc_init_device(int device);
 
 }  // namespace zkgpu
