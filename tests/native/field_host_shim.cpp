@@ -1,6 +1,10 @@
 // Host build of the library's field code for CPU tests (tests/test_field_host.py).
 #include <string.h>
 
+#include <array>
+#include <utility>
+
+#include "../../zk_gpu_amd/csrc/dif.hpp"
 #include "../../zk_gpu_amd/csrc/fr29.hpp"
 #include "../../zk_gpu_amd/csrc/host_field.hpp"
 
@@ -8,6 +12,46 @@ using namespace zkgpu;
 
 static fe ld(const uint32_t* p) { fe a; memcpy(a.l, p, sizeof(a.l)); return a; }
 static void st(uint32_t* p, const fe& a) { memcpy(p, a.l, sizeof(a.l)); }
+
+// sub<C> and its constant for every C below kMaxC (the DIF schedule
+// instantiates C up to MUL_IN_VMAX + 1)
+constexpr uint32_t kMaxC = 200;
+using SubFn = void (*)(const uint32_t*, const uint32_t*, uint32_t*);
+using KFn = uint32_t (*)(int);
+template <uint32_t C>
+void sub_c(const uint32_t* a, const uint32_t* b, uint32_t* o) { st(o, sub<C>(ld(a), ld(b))); }
+template <uint32_t C>
+uint32_t k_c(int i) { return SubConst<C>::k(i); }
+template <uint32_t... Cs>
+constexpr std::array<SubFn, sizeof...(Cs)> sub_table(std::integer_sequence<uint32_t, Cs...>) { return {{&sub_c<Cs>...}}; }
+template <uint32_t... Cs>
+constexpr std::array<KFn, sizeof...(Cs)> k_table(std::integer_sequence<uint32_t, Cs...>) { return {{&k_c<Cs>...}}; }
+static const auto SUB_TABLE = sub_table(std::make_integer_sequence<uint32_t, kMaxC>{});
+static const auto K_TABLE = k_table(std::make_integer_sequence<uint32_t, kMaxC>{});
+
+// The register DFT of the pass kernels, built for the host.
+template <int R, bool CHAIN>
+static void dft_host(const uint32_t* in, const uint32_t* roots, uint32_t root_log, uint32_t* out) {
+  alignas(16) uint32_t tab[12 * 64];
+  memcpy(tab, roots, sizeof(uint32_t) * 12 * (1u << root_log));
+  fe u[R];
+  for (int i = 0; i < R; ++i) u[i] = ld(in + 9 * i);
+  dft_regs<R, CHAIN>(u, (const uint4*)tab, root_log);
+  for (int i = 0; i < R; ++i) st(out + 9 * i, u[i]);
+}
+template <int R>
+static void schedule_host(uint32_t* out) {
+  // per butterfly: fx, fy, C; then fout per register
+  constexpr auto S = DifSched<R>::value;
+  constexpr int NB = DifSchedule<R>::NB;
+  int o = 0;
+  for (int b = 0; b < NB; ++b) {
+    out[o++] = S.fx[b];
+    out[o++] = S.fy[b];
+    out[o++] = S.c[b];
+  }
+  for (int i = 0; i < R; ++i) out[o++] = S.fout[i];
+}
 
 extern "C" {
 void shim_mont_mul(const uint32_t* a, const uint32_t* b, uint32_t* o) { st(o, mont_mul(ld(a), ld(b))); }
@@ -21,6 +65,38 @@ void shim_sub(uint32_t c, const uint32_t* a, const uint32_t* b, uint32_t* o) {
     default: memset(o, 0xff, 36);
   }
 }
+int shim_sub_any(uint32_t c, const uint32_t* a, const uint32_t* b, uint32_t* o) {
+  if (c >= kMaxC) return -1;
+  SUB_TABLE[c](a, b, o);
+  return 0;
+}
+int shim_subconst_any(uint32_t c, uint32_t* out) {
+  if (c >= kMaxC) return -1;
+  for (int i = 0; i < 9; ++i) out[i] = K_TABLE[c](i);
+  return 0;
+}
+int shim_dft(int R, int chain, const uint32_t* in, const uint32_t* roots, uint32_t root_log, uint32_t* out) {
+  if (root_log > 6) return -1;
+  switch (R * 2 + (chain ? 1 : 0)) {
+    case 4: dft_host<2, false>(in, roots, root_log, out); return 0;
+    case 5: dft_host<2, true>(in, roots, root_log, out); return 0;
+    case 8: dft_host<4, false>(in, roots, root_log, out); return 0;
+    case 9: dft_host<4, true>(in, roots, root_log, out); return 0;
+    case 16: dft_host<8, false>(in, roots, root_log, out); return 0;
+    case 17: dft_host<8, true>(in, roots, root_log, out); return 0;
+    default: return -1;
+  }
+}
+// the schedule of the radix-R network; returns the number of words written
+int shim_dif_schedule(int R, uint32_t* out) {
+  switch (R) {
+    case 2: schedule_host<2>(out); return 3 * DifSchedule<2>::NB + 2;
+    case 4: schedule_host<4>(out); return 3 * DifSchedule<4>::NB + 4;
+    case 8: schedule_host<8>(out); return 3 * DifSchedule<8>::NB + 8;
+    default: return -1;
+  }
+}
+void shim_mul_contract(uint32_t* out) { out[0] = (uint32_t)MUL_IN_MAX; out[1] = MUL_IN_VMAX; }
 void shim_normalize(uint32_t* a) { fe v = ld(a); normalize(v); st(a, v); }
 void shim_reduce_value(uint32_t* a) { fe v = ld(a); reduce_value(v); st(a, v); }
 void shim_pack(const uint32_t* a, uint32_t* w) { pack_words(w, ld(a)); }

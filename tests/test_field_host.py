@@ -145,3 +145,104 @@ def test_to_limbs261(shim):
         out = (ctypes.c_uint32 * 9)()
         shim.shim_to_limbs261(w, out)
         assert val(list(out)) == x * B261 % R
+
+
+def _sched(shim, R):
+    out = (ctypes.c_uint32 * 64)()
+    n = shim.shim_dif_schedule(R, out)
+    o = list(out)[:n]
+    nb = (n - R) // 3
+    bf = [tuple(o[3 * i:3 * i + 3]) for i in range(nb)]
+    return bf, o[3 * nb:]
+
+
+def test_subconst_every_c(shim):
+    """sub<C> constants for every C the radix-2/4/8 schedules may instantiate
+    (up to MUL_IN_VMAX + 1 = 161) and beyond: K = C*r, limbs 0..7 in
+    [2^29 - 1, 2^30) so that K[i] - b[i] never underflows for a normalized b
+    and adds less than 2^30 (the bound the schedule assumes)."""
+    shim.shim_subconst_any.restype = ctypes.c_int
+    for c in range(2, 200):
+        k = (ctypes.c_uint32 * 9)()
+        assert shim.shim_subconst_any(c, k) == 0
+        k = list(k)
+        assert val(k) == c * R, c
+        assert all(M29 <= v < (1 << 30) for v in k[:8]), c
+
+
+def test_schedule_constants_in_range(shim):
+    mc = (ctypes.c_uint32 * 2)()
+    shim.shim_mul_contract(mc)
+    vmax = mc[1]
+    for R in (2, 4, 8):
+        bf, fout = _sched(shim, R)
+        assert len(bf) == (R.bit_length() - 1) * R // 2
+        assert all(2 <= c <= vmax + 1 for _, _, c in bf), (R, bf)
+        assert all(f in (0, 1, 2) for f in fout)
+
+
+def test_sub_every_c_at_bounds(shim):
+    """a + C*r - b for every C in 2..161, with the subtrahend at the top of
+    its contract (normalized, V_b up to (C-1) r - 1) and the minuend's limbs
+    up to 2^32 - 1 - 2^30: exact, and every limb stays below 2^32."""
+    shim.shim_sub_any.restype = ctypes.c_int
+    rnd = random.Random(9)
+    for c in range(2, 162):
+        for t in range(40):
+            vb = (c - 1) * R - 1 if t == 0 else rnd.randrange((c - 1) * R)
+            yl = limbs(vb)
+            if t % 3 == 0:
+                xl = [(1 << 32) - 1 - (1 << 30)] * 8 + [rnd.randrange(1 << 31)]
+            else:
+                xl = limbs(rnd.randrange(80 * R))
+            o = (ctypes.c_uint32 * 9)()
+            assert shim.shim_sub_any(c, arr32(xl), arr32(yl), o) == 0
+            o = list(o)
+            assert val(o) == val(xl) + c * R - vb, (c, t)
+            assert all(o[i] <= xl[i] + (1 << 30) for i in range(8)), (c, t)
+
+
+def _root(logR):
+    w28 = pow(7, (R - 1) >> 28, R)
+    return pow(w28, 1 << (28 - logR), R)
+
+
+def _bitrev(v, bits):
+    return int(format(v, f"0{bits}b")[::-1], 2) if bits else 0
+
+
+@pytest.mark.parametrize("radix", [2, 4, 8])
+@pytest.mark.parametrize("chain", [0, 1])
+def test_dft_regs_worst_case_bounds(shim, radix, chain):
+    """The pass kernels' register DFT (natural order in, bit-reversed out),
+    built for the host, at the edge of its input contract (normalized limbs,
+    V < 6r): all inputs 6r - 1, limbs 0..7 all 2^29 - 1, alternating extremes,
+    and random values below 6r. Every output is congruent to the DFT and
+    meets the multiplier's operand contract (limbs <= 0xC0000000, V <= 160 r),
+    which the compile-time schedule promises."""
+    shim.shim_dft.restype = ctypes.c_int
+    logR = radix.bit_length() - 1
+    w = _root(logR)
+    tab = []
+    for e in range(radix):
+        tab += limbs(pow(w, e, R) * B261 % R) + [0, 0, 0]
+    roots = arr32(tab, 12 * radix)
+    top = 6 * R - 1
+    maxl = [M29] * 8 + [(top - sum(M29 << (29 * i) for i in range(8))) >> 232]
+    assert val(maxl) < 6 * R
+    rnd = random.Random(100 + radix + chain)
+    cases = [[top] * radix, [val(maxl)] * radix, [top if i % 2 else 0 for i in range(radix)],
+             [0] * (radix - 1) + [top]]
+    cases += [[rnd.randrange(6 * R) for _ in range(radix)] for _ in range(300)]
+    for xs in cases:
+        flat = []
+        for x in xs:
+            flat += maxl if x == val(maxl) else limbs(x)
+        out = (ctypes.c_uint32 * (9 * radix))()
+        assert shim.shim_dft(radix, chain, arr32(flat, 9 * radix), roots, logR, out) == 0
+        out = list(out)
+        for k in range(radix):
+            o = out[9 * _bitrev(k, logR):9 * _bitrev(k, logR) + 9]
+            want = sum(x * pow(w, i * k, R) for i, x in enumerate(xs)) % R
+            assert val(o) % R == want, (radix, chain, k, xs[:2])
+            assert all(v <= 0xC0000000 for v in o) and val(o) <= 160 * R, (radix, k)
