@@ -69,13 +69,30 @@ uint64_t parallel_coeff_to_extended_part_api(size_t device_id, uint64_t* req, co
                                              const uint64_t* twiddle, uint64_t n, uint64_t log_n,
                                              uint64_t twiddle_size, const uint64_t* c);
 
-/* replaces parallel_msm_api (src/api.rs:12, cuda/fft.cu:578-582): the G1 MSM
- * is outside this library; always returns ZKGPU_ERR_UNSUPPORTED. */
+/* replaces parallel_msm_api (src/api.rs:12, cuda/fft.cu:450-551,578-582):
+ * *acc <- sum_i data[i] * bases[i] over BN256 G1 (y^2 = x^3 + 3). data: length
+ * Fr scalars (4 u64 words each, halo2 Montgomery form, used as the canonical
+ * integers they stand for); bases: length affine points (GpuAffine: x then y,
+ * 4 u64 words each, Fq in Montgomery form x * 2^256 mod q; x = y = 0 is the
+ * identity, as in halo2); acc: one GpuProjective (x, y, z, 12 u64 words,
+ * Jacobian: affine = (x / z^2, y / z^3)), written with z = 1 (Montgomery one)
+ * or as all zero for the identity. Host buffers, synchronous. The reference's
+ * C definition takes a sixth argument that its Rust declaration does not pass;
+ * this one follows the Rust declaration. length <= 2^28. */
 uint64_t parallel_msm_api(size_t device_id, const uint64_t* data, const void* bases, uint64_t length, void* acc);
 
 /* ---------------------------------------------------------------------
  * Extensions (no counterpart in the reference FFI).
  * --------------------------------------------------------------------- */
+
+/* MSM on device-resident scalars (length x 32 bytes) and bases (length x 64
+ * bytes), as parallel_msm_api; acc is a host pointer to 12 words, written
+ * before the call returns (the call synchronises `stream`). */
+int zkgpu_msm_device(int device_id, const void* d_scalars, const void* d_bases, uint64_t length, uint64_t* acc,
+                     void* stream);
+/* Device workspace bytes of a length-point MSM; *window_bits gets the
+ * Pippenger window width c (signed digits, 2^(c-1) buckets per window). */
+uint64_t zkgpu_msm_workspace_bytes(uint64_t length, uint32_t* window_bits);
 
 /* Coset inverse on host buffers: req <- divisor * c^k * FFT_omega(req)[k];
  * with omega^-1, n^-1, g^-1 it inverts parallel_coeff_to_extended_part_api

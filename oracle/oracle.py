@@ -44,11 +44,16 @@ def lib():
             "oracle_gen": [P, U, U, U], "oracle_mul_many": [P, P, P, U], "oracle_add_many": [P, P, P, U],
             "oracle_sub_many": [P, P, P, U], "oracle_to_mont_many": [P, P, U], "oracle_from_mont_many": [P, P, U],
             "oracle_eval": [P, P, U, P],
+            "g1_oracle_msm": [P, P, P, U, I], "g1_oracle_to_affine": [P, P], "g1_oracle_mul_gen": [P, P],
+            "g1_oracle_arith_bases": [P, U, P, P], "g1_oracle_fq_mul": [P, P, P], "g1_oracle_fq_to_mont": [P, P],
+            "g1_oracle_fq_from_mont": [P, P],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
             fn.argtypes = args
-            fn.restype = U if name == "oracle_twiddle_lut" else None
+            fn.restype = U if name == "oracle_twiddle_lut" else (I if name.startswith("g1_oracle_") and
+                                                                  name[10:] in ("msm", "to_affine", "mul_gen")
+                                                                  else None)
         _lib = L
     return _lib
 
@@ -169,4 +174,52 @@ def run_op(op, a, log_n, g=7, threads=1):
 def evaluate(a, x):
     out = np.empty((1, 4), dtype=np.uint64)
     lib().oracle_eval(_p(out), _p(np.ascontiguousarray(a)), a.shape[0], _p(x))
+    return out
+
+
+# ---------------------------------------------------------------- G1 (BN256, y^2 = x^3 + 3 over Fq)
+
+Q_MOD = 0x30644E72E131A029B85045B68181585D97816A916871CA8D3C208C16D87CFD47
+Q_MONT_R = (1 << 256) % Q_MOD
+
+
+def fq_elem(x):
+    """canonical Fq integer -> (4,) Montgomery words"""
+    return to_limbs(x * Q_MONT_R % Q_MOD)
+
+
+def fq_value(a):
+    """(4,) Montgomery words -> canonical Fq integer"""
+    return from_limbs(a) * pow(Q_MONT_R, -1, Q_MOD) % Q_MOD
+
+
+def g1_msm(scalars, bases, threads=1):
+    """sum_i scalars[i] * bases[i]: scalars (n, 4) Fr Montgomery words, bases
+    (n, 8) affine Montgomery words (x then y). Returns None for the identity,
+    else the affine point as canonical integers (x, y)."""
+    out = np.zeros(8, dtype=np.uint64)
+    n = scalars.shape[0]
+    inf = lib().g1_oracle_msm(_p(out), _p(np.ascontiguousarray(scalars)), _p(np.ascontiguousarray(bases)), n,
+                              threads)
+    return None if inf else (fq_value(out[:4]), fq_value(out[4:]))
+
+
+def g1_to_affine(jac):
+    """(12,) Jacobian Montgomery words (x, y, z) -> None or canonical (x, y)."""
+    out = np.zeros(8, dtype=np.uint64)
+    inf = lib().g1_oracle_to_affine(_p(out), _p(np.ascontiguousarray(jac, dtype=np.uint64).reshape(12)))
+    return None if inf else (fq_value(out[:4]), fq_value(out[4:]))
+
+
+def g1_mul_gen(k):
+    """k G for the generator G = (1, 2): None or canonical (x, y)."""
+    out = np.zeros(8, dtype=np.uint64)
+    inf = lib().g1_oracle_mul_gen(_p(out), _p(to_limbs(k)))
+    return None if inf else (fq_value(out[:4]), fq_value(out[4:]))
+
+
+def g1_arith_bases(n, a, d):
+    """(n, 8) affine Montgomery words of (a + i d) G, i < n."""
+    out = np.zeros((n, 8), dtype=np.uint64)
+    lib().g1_oracle_arith_bases(_p(out), n, _p(to_limbs(a)), _p(to_limbs(d)))
     return out

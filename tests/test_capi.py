@@ -147,7 +147,7 @@ def test_drop_in_size_checks_before_any_device_work():
     assert L.parallel_fft_api(0, p, wp, None, 1000, 10, 0) == Z.ERR_INVALID       # n != 2^log_n
     assert L.parallel_fft_api(0, p, wp, None, 1 << 29, 29, 0) == Z.ERR_INVALID    # log_n too large
     assert L.parallel_ifft_api(0, p, wp, None, 1 << 10, 10, 0, None) == Z.ERR_INVALID  # no divisor
-    assert L.parallel_msm_api(0, None, None, 0, None) == Z.ERR_UNSUPPORTED
+    assert L.parallel_msm_api(0, None, None, 0, None) == Z.ERR_INVALID  # no result pointer
     assert L.zkgpu_ntt_std_device(0, None, None, 0, 0, None) == Z.ERR_INVALID
 
 

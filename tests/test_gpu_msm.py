@@ -1,0 +1,108 @@
+"""G1 multi-scalar multiplication on the device (parallel_msm_api and
+zkgpu_msm_device) against the golden vectors, the oracle (bit-serial Straus
+on the CPU, a different summation order), and the closed form
+sum_i s_i (a + i d) G = (sum_i s_i (a + i d) mod r) G. The result is a group
+element: compared as affine coordinates (the drop-in returns z = 1)."""
+import json
+import os
+import random
+
+import numpy as np
+import pytest
+
+import zk_gpu_amd as Z
+from oracle import oracle as O
+from tests.gpu_util import ptr
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "g1_vectors.json")
+R = O.R_MOD
+
+
+def _arr(rows):
+    return np.ascontiguousarray(np.array([[int(w, 16) for w in row] for row in rows], dtype=np.uint64))
+
+
+def _affine(acc):
+    """(3, 4) result words -> None (identity) or canonical (x, y); z must be 1 or 0"""
+    acc = np.asarray(acc, dtype=np.uint64).reshape(3, 4)
+    if not acc.any():
+        return None
+    assert O.fq_value(acc[2]) == 1  # z = 1 in Montgomery form
+    return O.g1_to_affine(acc.reshape(12))
+
+
+def _weighted_sum(scalars, a, d):
+    """sum_i s_i (a + i d) mod r, s_i the canonical values of the Montgomery
+    words: R^-1 sum_i m_i (a + i d) with m_i the stored integers"""
+    w = scalars.astype(object)
+    m = w[:, 0] + (w[:, 1] << 64) + (w[:, 2] << 128) + (w[:, 3] << 192)
+    idx = np.arange(scalars.shape[0], dtype=object)
+    return int((m * (a + idx * d)).sum()) % R * pow(O.MONT_R, -1, R) % R
+
+
+def _msm_host(scalars, bases):
+    acc = np.zeros((3, 4), dtype=np.uint64)
+    n = scalars.shape[0]
+    Z.check(Z.parallel_msm_api(0, scalars, bases, n, acc), "parallel_msm_api")
+    return _affine(acc)
+
+
+def test_golden_cases():
+    for c in json.load(open(GOLD))["msm"]:
+        got = _msm_host(_arr(c["scalars"]), _arr(c["bases"]))
+        want = None if c["expected"] is None else (int(c["expected"][0], 16), int(c["expected"][1], 16))
+        assert got == want, c["name"]
+
+
+def test_empty_and_zero():
+    acc = np.ones((3, 4), dtype=np.uint64)
+    assert Z.parallel_msm_api(0, np.zeros((1, 4), np.uint64), np.zeros((1, 8), np.uint64), 0, acc) == 0
+    assert not acc.any()  # the identity, as the reference leaves it
+    bases = O.g1_arith_bases(50, 3, 5)
+    assert _msm_host(np.zeros((50, 4), np.uint64), bases) is None
+
+
+@pytest.mark.parametrize("n", [1, 2, 31, 1000, 4096])
+def test_vs_oracle_with_edge_bases(n):
+    """random scalars over bases that include the identity, repeated bases
+    and a base next to its negation"""
+    rnd = random.Random(n)
+    bases = O.g1_arith_bases(n, rnd.randrange(1, R), rnd.randrange(1, R))
+    if n >= 4:
+        bases[1] = 0  # identity
+        bases[2] = bases[3]  # repeated base
+    if n >= 8:
+        bases[5] = bases[6]
+        y = O.fq_value(bases[6][4:])
+        bases[5][4:] = O.fq_elem(O.Q_MOD - y)  # -bases[6]
+    scalars = O.gen(4400 + n, n)
+    if n >= 10:
+        scalars[7] = 0
+        scalars[8] = O.elem(R - 1)
+    assert _msm_host(scalars, bases) == O.g1_msm(scalars, bases, threads=16)
+
+
+@pytest.mark.parametrize("log_n", [12, 16, 20])
+def test_closed_form_large(log_n):
+    n = 1 << log_n
+    a, d = 0x1234567 + log_n, 0x9876543210FEDCBA
+    bases = O.g1_arith_bases(n, a, d)
+    scalars = O.gen(4500 + log_n, n)
+    want = O.g1_mul_gen(_weighted_sum(scalars, a, d))
+    assert _msm_host(scalars, bases) == want
+
+
+def test_device_entry_point(torch_cuda):
+    torch = torch_cuda
+    n = 5000
+    bases = O.g1_arith_bases(n, 77, 1001)
+    scalars = O.gen(4600, n)
+    ds = torch.from_numpy(scalars.view(np.int64).reshape(-1)).to("cuda")
+    db = torch.from_numpy(bases.view(np.int64).reshape(-1)).to("cuda")
+    torch.cuda.synchronize()
+    acc = Z.msm_device(ptr(ds), ptr(db), n)
+    assert _affine(acc) == O.g1_mul_gen(_weighted_sum(scalars, 77, 1001))
+    # twice in a row on the same stream, the second smaller (workspace reuse)
+    acc2 = Z.msm_device(ptr(ds), ptr(db), 17)
+    assert _affine(acc2) == O.g1_msm(scalars[:17], bases[:17])

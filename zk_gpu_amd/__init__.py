@@ -52,6 +52,8 @@ SIGNATURES = {
     "zkgpu_ntt_std_device": (_I, [_I, _P, _P, _U32, _I, _P]),
     "zkgpu_ntt_device_noalloc": (_I, [_I, _P, _P, _U32, _I, _P, _P, _P, _P]),
     "zkgpu_noalloc_scratch_bytes": (_U64, [_P]),
+    "zkgpu_msm_device": (_I, [_I, _P, _P, _U64, _P, _P]),
+    "zkgpu_msm_workspace_bytes": (_U64, [_U64, _P]),
     "zkgpu_domain_constant": (_I, [_U32, _I, _P]),
     "zkgpu_prepare": (_I, [_I, _U32, _I, _P, _P, _P]),
     "zkgpu_fill_random": (_I, [_I, _P, _U64, _U64, _U64, _P]),
@@ -185,7 +187,17 @@ def parallel_coeff_to_extended_part_api(device_id, req, omega, twiddle, n, log_n
 
 
 def parallel_msm_api(device_id, data, bases, length, acc):
-    return int(lib().parallel_msm_api(device_id, _ptr(data), bases, length, acc))
+    """acc (numpy (3, 4) uint64: x, y, z) <- sum_i data[i] * bases[i] over G1;
+    data (length, 4) uint64 Fr Montgomery words, bases (length, 8) uint64 (x
+    then y, Fq Montgomery words) (reference cuda/fft.cu:450-551)."""
+    return int(lib().parallel_msm_api(device_id, _ptr(data), _ptr(bases), length, _ptr(acc)))
+
+
+def msm_device(d_scalars, d_bases, length, stream=None, device_id=0):
+    """MSM on device pointers; returns the (3, 4) uint64 result (x, y, z)."""
+    acc = np.zeros((3, 4), dtype=np.uint64)
+    check(lib().zkgpu_msm_device(device_id, d_scalars, d_bases, length, _ptr(acc), stream), "zkgpu_msm_device")
+    return acc
 
 
 def zkgpu_coset_ifft_api(device_id, req, omega, n, log_n, divisor, c):

@@ -17,7 +17,8 @@ LIB = os.path.join(HERE, "libzkgpu_ntt.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("ZKGPU_ARCH", "gfx950")
 
-SOURCES = ["ntt_kernels.hip", "noalloc.hip", "batch.cpp", "engine.cpp", "capi.cpp", "serve.cpp", "host_stage.cpp"]
+SOURCES = ["ntt_kernels.hip", "noalloc.hip", "msm_kernels.hip", "batch.cpp", "engine.cpp", "capi.cpp", "serve.cpp",
+           "host_stage.cpp", "msm.cpp"]
 # packed-FP32 instructions are disabled for all device code (they are not
 # used by this integer code, and some of their forms are unreliable)
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall", "-Wno-unused-function",

@@ -151,15 +151,6 @@ uint64_t parallel_coeff_to_extended_part_api(size_t device_id, uint64_t* req, co
   return (uint64_t)host_call(device_id, req, log_n, OP_COSET_FWD, omega, nullptr, c);
 }
 
-uint64_t parallel_msm_api(size_t device_id, const uint64_t* data, const void* bases, uint64_t length, void* acc) {
-  (void)device_id;
-  (void)data;
-  (void)bases;
-  (void)length;
-  (void)acc;
-  return ZK_ERR_UNSUPPORTED;
-}
-
 uint64_t zkgpu_coset_ifft_api(size_t device_id, uint64_t* req, const uint64_t* omega, uint64_t n, uint64_t log_n,
                               const uint64_t* divisor, const uint64_t* c) {
   if (!size_ok(n, log_n)) return ZK_ERR_INVALID;

@@ -156,6 +156,7 @@ enum WsSlot : int {
   WS_PLANAR = 0,       // inter-pass workspace of a single transform
   WS_BATCH_ARENA = 1,  // workspaces of an uncaptured batch
   WS_BATCH_META = 2,   // descriptors and counters of an uncaptured batch
+  WS_MSM = 3,          // workspace of a device-resident MSM
 };
 
 // One (stream, slot) workspace. Its mutex is held by a WsLease for as long as

@@ -1,0 +1,336 @@
+// G1 multi-scalar multiplication: plan, device workspace, launches, and the
+// C-ABI entry points (parallel_msm_api of the reference, cuda/fft.cu:450-551,
+// 578-582, and a device-resident variant).
+//
+// The device computes one sum per window (msm_kernels.hip); the host combines
+// the W window sums by Horner's rule (c doublings and one addition per
+// window, a few hundred point operations on 4 x 64-bit Montgomery words) and
+// returns the result with Z = 1 (the identity as X = Y = Z = 0, as the
+// reference leaves it).
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "../../include/zkgpu_ntt.h"
+#include "engine.hpp"
+
+namespace zkgpu {
+
+hipError_t launch_msm_digits(const void* scalars, const void* bases, uint32_t n, uint32_t c, uint32_t windows,
+                             void* blimbs, int32_t* digits, uint32_t* counts, hipStream_t s);
+hipError_t launch_msm_scan(const uint32_t* counts, uint32_t m, uint32_t* sums, uint32_t* offsets, uint32_t* cursor,
+                           hipStream_t s);
+hipError_t launch_msm_scatter(const int32_t* digits, uint32_t n, uint32_t c, uint32_t windows, uint32_t* cursor,
+                              uint32_t* list, hipStream_t s);
+hipError_t launch_msm_buckets(const void* blimbs, const uint32_t* offsets, const uint32_t* counts,
+                              const uint32_t* list, uint32_t nbuckets, void* buckets, hipStream_t s);
+hipError_t launch_msm_chunks(const void* buckets, uint32_t half, uint32_t run, uint32_t runs_per_window,
+                             uint32_t windows, void* parts, hipStream_t s);
+hipError_t launch_msm_windows(const void* parts, uint32_t runs_per_window, uint32_t windows, void* out,
+                              hipStream_t s);
+size_t msm_base_limb_bytes();
+size_t msm_point_limb_bytes();
+
+namespace {
+
+// ---------------------------------------------------------------- host Fq
+
+// Fq elements as four little-endian u64 words in Montgomery form (R = 2^256),
+// CIOS multiply over unsigned __int128.
+struct Q {
+  uint64_t w[4];
+};
+constexpr Q QMOD = {{0x3c208c16d87cfd47ULL, 0x97816a916871ca8dULL, 0xb85045b68181585dULL, 0x30644e72e131a029ULL}};
+constexpr uint64_t QINV = 0x87d20782e4866389ULL;  // -q^-1 mod 2^64
+constexpr Q QZERO = {{0, 0, 0, 0}};
+constexpr Q QONE = {{0xd35d438dc58f0d9dULL, 0x0a78eb28f5c70b3dULL, 0x666ea36f7879462cULL, 0x0e0a77c19a07df2fULL}};
+
+bool q_geq(const Q& a, const Q& b) {
+  for (int i = 3; i >= 0; --i)
+    if (a.w[i] != b.w[i]) return a.w[i] > b.w[i];
+  return true;
+}
+void q_sub_raw(Q& a, const Q& b) {
+  unsigned __int128 bw = 0;
+  for (int i = 0; i < 4; ++i) {
+    const unsigned __int128 d = (unsigned __int128)a.w[i] - b.w[i] - bw;
+    a.w[i] = (uint64_t)d;
+    bw = (d >> 64) & 1;
+  }
+}
+Q q_add(const Q& a, const Q& b) {
+  Q o;
+  unsigned __int128 c = 0;
+  for (int i = 0; i < 4; ++i) {
+    const unsigned __int128 s = (unsigned __int128)a.w[i] + b.w[i] + c;
+    o.w[i] = (uint64_t)s;
+    c = s >> 64;
+  }
+  if (c || q_geq(o, QMOD)) q_sub_raw(o, QMOD);
+  return o;
+}
+Q q_sub(const Q& a, const Q& b) {
+  if (q_geq(a, b)) {
+    Q o = a;
+    q_sub_raw(o, b);
+    return o;
+  }
+  Q o = a;  // a + q < 2^255: no carry out of the top word
+  unsigned __int128 c = 0;
+  for (int i = 0; i < 4; ++i) {
+    const unsigned __int128 s = (unsigned __int128)o.w[i] + QMOD.w[i] + c;
+    o.w[i] = (uint64_t)s;
+    c = s >> 64;
+  }
+  q_sub_raw(o, b);
+  return o;
+}
+Q q_mul(const Q& a, const Q& b) {
+  uint64_t t[6] = {0, 0, 0, 0, 0, 0};
+  for (int i = 0; i < 4; ++i) {
+    uint64_t c = 0;
+    for (int j = 0; j < 4; ++j) {
+      const unsigned __int128 p = (unsigned __int128)a.w[i] * b.w[j] + t[j] + c;
+      t[j] = (uint64_t)p;
+      c = (uint64_t)(p >> 64);
+    }
+    unsigned __int128 s = (unsigned __int128)t[4] + c;
+    t[4] = (uint64_t)s;
+    t[5] = (uint64_t)(s >> 64);
+    const uint64_t m = t[0] * QINV;
+    unsigned __int128 p = (unsigned __int128)m * QMOD.w[0] + t[0];
+    c = (uint64_t)(p >> 64);
+    for (int j = 1; j < 4; ++j) {
+      p = (unsigned __int128)m * QMOD.w[j] + t[j] + c;
+      t[j - 1] = (uint64_t)p;
+      c = (uint64_t)(p >> 64);
+    }
+    s = (unsigned __int128)t[4] + c;
+    t[3] = (uint64_t)s;
+    t[4] = t[5] + (uint64_t)(s >> 64);
+  }
+  Q o = {{t[0], t[1], t[2], t[3]}};
+  if (t[4] || q_geq(o, QMOD)) q_sub_raw(o, QMOD);
+  return o;
+}
+bool q_is_zero(const Q& a) { return (a.w[0] | a.w[1] | a.w[2] | a.w[3]) == 0; }
+bool q_eq(const Q& a, const Q& b) { return memcmp(a.w, b.w, 32) == 0; }
+Q q_inv(const Q& a) {  // a^(q-2)
+  Q e = QMOD;
+  e.w[0] -= 2;
+  Q acc = QONE;
+  for (int i = 3; i >= 0; --i)
+    for (int k = 63; k >= 0; --k) {
+      acc = q_mul(acc, acc);
+      if ((e.w[i] >> k) & 1) acc = q_mul(acc, a);
+    }
+  return acc;
+}
+
+struct Jac {
+  Q x, y, z;
+};
+Jac j_identity() { return Jac{QZERO, QZERO, QZERO}; }
+bool j_is_identity(const Jac& p) { return q_is_zero(p.z); }
+Jac j_double(const Jac& p) {  // dbl-2009-l
+  if (j_is_identity(p)) return p;
+  const Q A = q_mul(p.x, p.x), B = q_mul(p.y, p.y), C = q_mul(B, B);
+  const Q t = q_add(p.x, B);
+  Q D = q_sub(q_sub(q_mul(t, t), A), C);
+  D = q_add(D, D);
+  const Q E = q_add(q_add(A, A), A);
+  const Q F = q_mul(E, E);
+  Jac o;
+  o.x = q_sub(F, q_add(D, D));
+  Q C8 = q_add(C, C);
+  C8 = q_add(C8, C8);
+  C8 = q_add(C8, C8);
+  o.y = q_sub(q_mul(E, q_sub(D, o.x)), C8);
+  const Q yz = q_mul(p.y, p.z);
+  o.z = q_add(yz, yz);
+  return o;
+}
+Jac j_add(const Jac& p, const Jac& q) {  // add-2007-bl
+  if (j_is_identity(p)) return q;
+  if (j_is_identity(q)) return p;
+  const Q Z1Z1 = q_mul(p.z, p.z), Z2Z2 = q_mul(q.z, q.z);
+  const Q U1 = q_mul(p.x, Z2Z2), U2 = q_mul(q.x, Z1Z1);
+  const Q S1 = q_mul(p.y, q_mul(q.z, Z2Z2)), S2 = q_mul(q.y, q_mul(p.z, Z1Z1));
+  const Q H = q_sub(U2, U1);
+  Q R = q_sub(S2, S1);
+  R = q_add(R, R);
+  if (q_is_zero(H)) return q_is_zero(R) ? j_double(p) : j_identity();
+  const Q H2 = q_add(H, H);
+  const Q I = q_mul(H2, H2), J = q_mul(H, I), V = q_mul(U1, I);
+  Jac o;
+  o.x = q_sub(q_sub(q_mul(R, R), J), q_add(V, V));
+  const Q SJ = q_mul(S1, J);
+  o.y = q_sub(q_mul(R, q_sub(V, o.x)), q_add(SJ, SJ));
+  const Q zz = q_add(p.z, q.z);
+  o.z = q_mul(q_sub(q_sub(q_mul(zz, zz), Z1Z1), Z2Z2), H);
+  return o;
+}
+
+}  // namespace
+
+// Window size and workspace of an n-point MSM.
+struct MsmPlan {
+  uint32_t n = 0, c = 0, windows = 0, half = 0, run = 0, runs = 0;
+  size_t off_blimbs = 0, off_digits = 0, off_list = 0, off_counts = 0, off_offsets = 0, off_cursor = 0,
+         off_sums = 0, off_buckets = 0, off_parts = 0, off_wout = 0, bytes = 0;
+};
+
+static size_t up256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+MsmPlan msm_plan(uint32_t n) {
+  MsmPlan p;
+  p.n = n;
+  uint32_t lg = 0;
+  while ((1ull << (lg + 1)) <= n) ++lg;
+  // about n / 16 buckets per window once n is large: bucket sums dominate
+  p.c = std::max(4u, std::min(16u, lg > 4 ? lg - 4 : 4u));
+  p.windows = (255 + p.c - 1) / p.c;  // signed digits: W * c >= 255 bits cover every scalar below 2^254
+  p.half = 1u << (p.c - 1);
+  p.run = std::min(16u, p.half);
+  p.runs = p.half / p.run;
+  const size_t nb = (size_t)p.windows * p.half;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t o = off;
+    off += up256(bytes);
+    return o;
+  };
+  p.off_blimbs = take((size_t)n * msm_base_limb_bytes());
+  p.off_digits = take((size_t)p.windows * n * 4);
+  p.off_list = take((size_t)p.windows * n * 4);
+  p.off_counts = take(nb * 4);
+  p.off_offsets = take(nb * 4);
+  p.off_cursor = take(nb * 4);
+  p.off_sums = take(((nb + 1023) / 1024) * 4);
+  p.off_buckets = take(nb * msm_point_limb_bytes());
+  p.off_parts = take((size_t)p.windows * p.runs * msm_point_limb_bytes());
+  p.off_wout = take((size_t)p.windows * 96);
+  p.bytes = off;
+  return p;
+}
+
+// Enqueue the device part of an MSM on s; the window sums (W x 96 bytes of
+// halo2 words X, Y, Z) land at ws + off_wout.
+int msm_enqueue(const MsmPlan& p, const void* d_scalars, const void* d_bases, void* ws, hipStream_t s) {
+  char* b = (char*)ws;
+  const uint32_t nb = p.windows * p.half;
+  uint32_t* counts = (uint32_t*)(b + p.off_counts);
+  hipError_t e = launch_reset_counters(counts, nb, s);
+  if (e == hipSuccess)
+    e = launch_msm_digits(d_scalars, d_bases, p.n, p.c, p.windows, b + p.off_blimbs, (int32_t*)(b + p.off_digits),
+                          counts, s);
+  if (e == hipSuccess)
+    e = launch_msm_scan(counts, nb, (uint32_t*)(b + p.off_sums), (uint32_t*)(b + p.off_offsets),
+                        (uint32_t*)(b + p.off_cursor), s);
+  if (e == hipSuccess)
+    e = launch_msm_scatter((const int32_t*)(b + p.off_digits), p.n, p.c, p.windows, (uint32_t*)(b + p.off_cursor),
+                           (uint32_t*)(b + p.off_list), s);
+  if (e == hipSuccess)
+    e = launch_msm_buckets(b + p.off_blimbs, (const uint32_t*)(b + p.off_offsets), counts,
+                           (const uint32_t*)(b + p.off_list), nb, b + p.off_buckets, s);
+  if (e == hipSuccess) e = launch_msm_chunks(b + p.off_buckets, p.half, p.run, p.runs, p.windows, b + p.off_parts, s);
+  if (e == hipSuccess) e = launch_msm_windows(b + p.off_parts, p.runs, p.windows, b + p.off_wout, s);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return ZK_ERR_HIP;
+  }
+  return ZK_OK;
+}
+
+// Horner over the window sums (halo2 words, 12 u64 each); out: X, Y, Z with
+// Z = 1, or all zero for the identity.
+void msm_combine(const MsmPlan& p, const uint64_t* wsum, uint64_t out[12]) {
+  Jac acc = j_identity();
+  for (uint32_t w = p.windows; w-- > 0;) {
+    for (uint32_t k = 0; k < p.c; ++k) acc = j_double(acc);
+    Jac s;
+    memcpy(s.x.w, wsum + 12 * w, 32);
+    memcpy(s.y.w, wsum + 12 * w + 4, 32);
+    memcpy(s.z.w, wsum + 12 * w + 8, 32);
+    acc = j_add(acc, s);
+  }
+  memset(out, 0, 96);
+  if (j_is_identity(acc)) return;
+  const Q zi = q_inv(acc.z), zi2 = q_mul(zi, zi), zi3 = q_mul(zi2, zi);
+  const Q x = q_mul(acc.x, zi2), y = q_mul(acc.y, zi3);
+  memcpy(out, x.w, 32);
+  memcpy(out + 4, y.w, 32);
+  memcpy(out + 8, QONE.w, 32);
+}
+
+}  // namespace zkgpu
+
+using namespace zkgpu;
+
+extern "C" {
+
+int zkgpu_msm_device(int device_id, const void* d_scalars, const void* d_bases, uint64_t length, uint64_t* acc,
+                     void* stream) {
+  if (!acc || length > (1ull << 28)) return ZK_ERR_INVALID;
+  if (length && (!d_scalars || !d_bases)) return ZK_ERR_INVALID;
+  memset(acc, 0, 96);
+  if (length == 0) return ZK_OK;
+  int st = ZK_OK;
+  Engine* e = Engine::get(device_id, &st);
+  if (!e) return st;
+  hipStream_t s = (hipStream_t)stream;
+  const MsmPlan p = msm_plan((uint32_t)length);
+  WsLease lease = e->lease_workspace(s, p.bytes, WS_MSM, &st);
+  if (st) return st;
+  st = msm_enqueue(p, d_scalars, d_bases, lease.ptr(), s);
+  std::vector<uint64_t> wsum((size_t)p.windows * 12);
+  if (st == ZK_OK &&
+      hipMemcpyAsync(wsum.data(), (char*)lease.ptr() + p.off_wout, wsum.size() * 8, hipMemcpyDeviceToHost, s) !=
+          hipSuccess)
+    st = ZK_ERR_HIP;
+  if (hipStreamSynchronize(s) != hipSuccess && st == ZK_OK) st = ZK_ERR_HIP;
+  if (st) return st;
+  msm_combine(p, wsum.data(), acc);
+  return ZK_OK;
+}
+
+uint64_t parallel_msm_api(size_t device_id, const uint64_t* data, const void* bases, uint64_t length, void* acc) {
+  if (!acc || length > (1ull << 28)) return ZK_ERR_INVALID;
+  if (length && (!data || !bases)) return ZK_ERR_INVALID;
+  if (length == 0) {
+    memset(acc, 0, 96);
+    return ZK_OK;
+  }
+  int st = ZK_OK;
+  Engine* e = Engine::get((int)device_id, &st);
+  if (!e) return (uint64_t)st;
+  const size_t sbytes = (size_t)length * 32, bbytes = (size_t)length * 64;
+  const MsmPlan p = msm_plan((uint32_t)length);
+  HostLease ctx(e, sbytes + bbytes, 0, p.bytes);
+  if (ctx.status()) return (uint64_t)ctx.status();
+  hipStream_t s = ctx->stream;
+  char* io = (char*)ctx->io->ptr;
+  st = stage_h2d((int)device_id, s, io, data, sbytes);
+  if (st == ZK_OK) st = stage_h2d((int)device_id, s, io + sbytes, bases, bbytes);
+  if (st == ZK_OK) st = msm_enqueue(p, io, io + sbytes, ctx->aux->ptr, s);
+  std::vector<uint64_t> wsum((size_t)p.windows * 12);
+  if (st == ZK_OK &&
+      hipMemcpyAsync(wsum.data(), (char*)ctx->aux->ptr + p.off_wout, wsum.size() * 8, hipMemcpyDeviceToHost, s) !=
+          hipSuccess)
+    st = ZK_ERR_HIP;
+  if (hipStreamSynchronize(s) != hipSuccess && st == ZK_OK) st = ZK_ERR_HIP;
+  if (st) return (uint64_t)st;
+  msm_combine(p, wsum.data(), (uint64_t*)acc);
+  return ZK_OK;
+}
+
+uint64_t zkgpu_msm_workspace_bytes(uint64_t length, uint32_t* window_bits) {
+  if (length == 0 || length > (1ull << 28)) return 0;
+  const MsmPlan p = msm_plan((uint32_t)length);
+  if (window_bits) *window_bits = p.c;
+  return p.bytes;
+}
+
+}  // extern "C"
