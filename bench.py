@@ -295,6 +295,89 @@ def bench_parts(torch, Z, dev, stream, log_n=20, parts=4, reps=5):
     return res
 
 
+# The reference's only recorded MSM timings (a log line in a comment,
+# cuda/fft.cu:447-448): commit_lagrange at k = 24 on its GPU path, hardware
+# unspecified.
+REF_MSM_K24_S = {"base_split=128": 1031.868653704, "base_split=256": 951.633471051}
+
+
+def bench_msm(torch, Z, dev, stream, p_mul, log_ns=(20, 24)):
+    """G1 MSM (zkgpu_msm_device) on device-resident synthetic inputs: random
+    Fr scalars, bases k_i G. Wall time per call (the call returns the result
+    on the host: launches, window sums, host Horner); kernel durations from
+    events around each MSM launch. The bucket kernel's algorithmic work is
+    n * W mixed additions (Pippenger, W windows of c bits), each 11 Fq products
+    (madd-2007-bl: 7M + 4S) of 136 textbook multiply-adds."""
+    import ctypes
+    sptr = stream.cuda_stream
+    out = []
+    for log_n in log_ns:
+        n = 1 << log_n
+        sc = torch.empty(n * 4, dtype=torch.int64, device=dev)
+        bs = torch.empty(n * 8, dtype=torch.int64, device=dev)
+        Z.fill_random(sc.data_ptr(), n, 7000 + log_n, stream=sptr)
+        Z.fill_random_points(bs.data_ptr(), n, 8000 + log_n, stream=sptr)
+        torch.cuda.synchronize()
+        cbits = ctypes.c_uint32(0)
+        ws = Z.lib().zkgpu_msm_workspace_bytes(n, ctypes.byref(cbits))
+        c = cbits.value
+        windows = (255 + c - 1) // c
+        Z.msm_device(sc.data_ptr(), bs.data_ptr(), n, stream=sptr)  # warm-up (workspace, clocks)
+        reps = 5 if log_n <= 20 else 3
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            Z.msm_device(sc.data_ptr(), bs.data_ptr(), n, stream=sptr)
+        wall = (time.perf_counter() - t0) / reps
+        Z.pass_timing(True)
+        Z.pass_times()
+        Z.msm_device(sc.data_ptr(), bs.data_ptr(), n, stream=sptr)
+        kt = Z.pass_times()
+        Z.pass_timing(False)
+        kernels = {k: v[1] / v[0] * 1e3 for k, v in kt.items()}  # us per launch
+        madds = n * windows
+        mads = madds * 11 * MADS_PER_MODMUL
+        tb = kernels.get("msm_buckets_kernel", 0) * 1e-6
+        ach = mads / tb / 1e12 if tb > 0 else None
+        out.append({"log_n": log_n, "points": n, "window_bits": c, "windows": windows, "workspace_bytes": ws,
+                    "ms": wall * 1e3, "points_per_s": n / wall, "kernels_us": kernels,
+                    "roofline": {"bound": "valu", "kernel": "msm_buckets_kernel", "achieved": ach,
+                                 "peak": PEAK_MAD_TOPS, "unit": "Tmad/s",
+                                 "frac": ach / PEAK_MAD_TOPS if ach else None,
+                                 "frac_measured": ach / p_mul if ach else None,
+                                 "work": f"{madds} mixed additions x 11 Fq products x 136 multiply-adds"}})
+        del sc, bs
+    res = {"config": "G1 MSM (BN256, y^2 = x^3 + 3), random Fr scalars, synthetic bases k_i G, device resident",
+           "sizes": out, "reference_logged_k24_s": REF_MSM_K24_S,
+           "reference_note": "the reference's own log line for commit_lagrange at k=24 (cuda/fft.cu:447-448), "
+                             "hardware unspecified; not a same-hardware comparison"}
+    k24 = [o for o in out if o["log_n"] == 24]
+    if k24:
+        res["speedup_vs_reference_logged_k24"] = REF_MSM_K24_S["base_split=256"] / (k24[0]["ms"] * 1e-3)
+    return res
+
+
+def cpu_baseline_msm(budget_s=10.0):
+    """The oracle's MSM (bit-serial Straus, every host core) on 2^14 points."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle import oracle as O  # the checker, used here only as the timed CPU path
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    n = 1 << 14
+    bases = O.g1_arith_bases(n, 12345, 6789)
+    scalars = O.gen(9100, n)
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        O.g1_msm(scalars, bases, threads=cores)
+        reps += 1
+        if time.perf_counter() - t0 > budget_s or reps >= 5:
+            break
+    el = (time.perf_counter() - t0) / reps
+    return {"value": n / el, "unit": "points/s", "cores": cores, "kind": "port", "cpu_model": cpu_model(),
+            "sample": f"{reps} MSMs of 2^14 points (oracle: bit-serial Straus in C, {cores} threads)"}
+
+
 def _serve_spec(count, seed=5, lo=14, hi=22):
     import random
     rnd = random.Random(seed)
@@ -455,6 +538,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-host", action="store_true", help="skip the host-buffer entry point timing")
     ap.add_argument("--no-parts", action="store_true", help="skip the extended-domain parts timing")
+    ap.add_argument("--no-msm", action="store_true", help="skip the G1 MSM timing")
     ap.add_argument("--serve-jobs", type=int, default=10_000)
     ap.add_argument("--copy-probe", action="store_true",
                     help="copy 256 MiB once on the device first: a kernel of known traffic for "
@@ -622,8 +706,16 @@ def main():
     if not args.no_serve:
         res["served"] = bench_serving(torch, Z, dev, stream, count=args.serve_jobs)
 
+    if not args.no_msm:
+        res["msm"] = bench_msm(torch, Z, dev, stream, p_mul)
+
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline()
+        if not args.no_msm:
+            res["msm"]["cpu_baseline"] = cpu_baseline_msm()
+            m20 = [o for o in res["msm"]["sizes"] if o["log_n"] == 20]
+            if m20:
+                res["msm"]["speedup_vs_cpu_2^20"] = m20[0]["points_per_s"] / res["msm"]["cpu_baseline"]["value"]
     print(json.dumps(res), flush=True)
 
 

@@ -90,6 +90,11 @@ uint64_t parallel_msm_api(size_t device_id, const uint64_t* data, const void* ba
  * before the call returns (the call synchronises `stream`). */
 int zkgpu_msm_device(int device_id, const void* d_scalars, const void* d_bases, uint64_t length, uint64_t* acc,
                      void* stream);
+/* Seeded synthetic bases: point i of d_out (64 bytes, affine x then y, Fq
+ * Montgomery words) = k G for the generator G = (1, 2), k a nonzero 64-bit
+ * integer derived from (seed, start + i) by SplitMix64. Asynchronous. */
+int zkgpu_fill_random_points(int device_id, void* d_out, uint64_t count, uint64_t seed, uint64_t start,
+                             void* stream);
 /* Device workspace bytes of a length-point MSM; *window_bits gets the
  * Pippenger window width c (signed digits, 2^(c-1) buckets per window). */
 uint64_t zkgpu_msm_workspace_bytes(uint64_t length, uint32_t* window_bits);
@@ -308,10 +313,11 @@ uint64_t zkgpu_set_host_budget(int device_id, uint64_t bytes);
  * the next `count` fail as if the device were out of memory. */
 int zkgpu_debug_fail_allocs(uint32_t skip, uint32_t count);
 
-/* Per-kernel timing of the pass kernels, for measurement: while enabled,
- * every pass that a single transform (zkgpu_ntt_device and the entry points
- * built on it) launches carries a start and a stop event tied to its dispatch
- * on the caller's stream (hipExtLaunchKernel). zkgpu_pass_times waits for
+/* Per-kernel timing, for measurement: while enabled, every pass that a single
+ * transform (zkgpu_ntt_device and the entry points built on it) launches
+ * carries a start and a stop event tied to its dispatch on the caller's
+ * stream (hipExtLaunchKernel), and each MSM kernel launch is bracketed by
+ * events on its stream. zkgpu_pass_times waits for
  * the recorded launches and reports, per kernel, the launch count and the
  * summed duration, then forgets them; *count gets the number of kernels
  * (up to max_out entries are written). */

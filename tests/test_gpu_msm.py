@@ -106,3 +106,27 @@ def test_device_entry_point(torch_cuda):
     # twice in a row on the same stream, the second smaller (workspace reuse)
     acc2 = Z.msm_device(ptr(ds), ptr(db), 17)
     assert _affine(acc2) == O.g1_msm(scalars[:17], bases[:17])
+
+
+def _mix64(z):
+    M = (1 << 64) - 1
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+    return z ^ (z >> 31)
+
+
+def test_fill_random_points(torch_cuda):
+    """Synthetic bases: point i = k_i G with k_i from SplitMix64 (the rule in
+    include/zkgpu_ntt.h), checked against the oracle's k G."""
+    torch = torch_cuda
+    n, seed, start = 300, 42, 1000
+    d = torch.empty(n * 8, dtype=torch.int64, device="cuda")
+    Z.fill_random_points(ptr(d), n, seed, start)
+    torch.cuda.synchronize()
+    pts = d.cpu().numpy().view(np.uint64).reshape(n, 8)
+    M = (1 << 64) - 1
+    for i in (0, 1, 150, n - 1):
+        k = _mix64((seed * 0xD1B54A32D192ED03 + (start + i) * 0x9E3779B97F4A7C15) & M) | 1
+        assert (O.fq_value(pts[i][:4]), O.fq_value(pts[i][4:])) == O.g1_mul_gen(k), i
+    s = O.gen(4700, n)
+    assert _msm_host(s, np.ascontiguousarray(pts)) == O.g1_msm(s, pts, threads=8)

@@ -54,6 +54,7 @@ SIGNATURES = {
     "zkgpu_noalloc_scratch_bytes": (_U64, [_P]),
     "zkgpu_msm_device": (_I, [_I, _P, _P, _U64, _P, _P]),
     "zkgpu_msm_workspace_bytes": (_U64, [_U64, _P]),
+    "zkgpu_fill_random_points": (_I, [_I, _P, _U64, _U64, _U64, _P]),
     "zkgpu_domain_constant": (_I, [_U32, _I, _P]),
     "zkgpu_prepare": (_I, [_I, _U32, _I, _P, _P, _P]),
     "zkgpu_fill_random": (_I, [_I, _P, _U64, _U64, _U64, _P]),
@@ -191,6 +192,11 @@ def parallel_msm_api(device_id, data, bases, length, acc):
     data (length, 4) uint64 Fr Montgomery words, bases (length, 8) uint64 (x
     then y, Fq Montgomery words) (reference cuda/fft.cu:450-551)."""
     return int(lib().parallel_msm_api(device_id, _ptr(data), _ptr(bases), length, _ptr(acc)))
+
+
+def fill_random_points(d_out, count, seed, start=0, stream=None, device_id=0):
+    """Synthetic affine bases k_i G (64 bytes each) at device pointer d_out."""
+    check(lib().zkgpu_fill_random_points(device_id, d_out, count, seed, start, stream), "zkgpu_fill_random_points")
 
 
 def msm_device(d_scalars, d_bases, length, stream=None, device_id=0):

@@ -298,6 +298,33 @@ void Engine::set_pass_timing(bool on) {
   timing_.store(on);
 }
 
+void Engine::timed_events(hipEvent_t* a, hipEvent_t* b) {
+  std::lock_guard<std::mutex> g(timing_mu_);
+  for (hipEvent_t* e : {a, b}) {
+    if (!timing_pool_.empty()) {
+      *e = timing_pool_.back();
+      timing_pool_.pop_back();
+    } else if (hipEventCreateWithFlags(e, hipEventDisableSystemFence) != hipSuccess) {
+      (void)hipGetLastError();
+      *e = nullptr;
+    }
+  }
+}
+
+void Engine::finish_timed(hipEvent_t a, hipEvent_t b, const char* name) {
+  std::lock_guard<std::mutex> g(timing_mu_);
+  if (!a || !b) {
+    if (a) timing_pool_.push_back(a);
+    if (b) timing_pool_.push_back(b);
+    return;
+  }
+  TimedPass tp;
+  tp.a = a;
+  tp.b = b;
+  tp.name = name;
+  timed_.push_back(tp);
+}
+
 int Engine::pass_times(std::vector<KernelTime>& out) {
   out.clear();
   std::lock_guard<std::mutex> g(timing_mu_);

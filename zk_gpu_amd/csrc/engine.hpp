@@ -232,7 +232,12 @@ class Engine {
     double total_ms = 0;
   };
   void set_pass_timing(bool on);
+  bool pass_timing() const { return timing_.load(std::memory_order_relaxed); }
   int pass_times(std::vector<KernelTime>& out);
+  // Timing of other launches (the MSM's): a start and a stop event recorded
+  // around one launch on s; finish_timed pairs them with the kernel's name.
+  void timed_events(hipEvent_t* a, hipEvent_t* b);
+  void finish_timed(hipEvent_t a, hipEvent_t b, const char* name);
 
   // Host-call contexts (host-buffer entry points).
   int acquire_host(size_t io_bytes, size_t planar_bytes, size_t aux_bytes, HostCtx** out);

@@ -32,6 +32,7 @@ hipError_t launch_msm_windows(const void* parts, uint32_t runs_per_window, uint3
                               hipStream_t s);
 size_t msm_base_limb_bytes();
 size_t msm_point_limb_bytes();
+hipError_t launch_fill_random_points(void* out, uint64_t count, uint64_t seed, uint64_t start, hipStream_t s);
 
 namespace {
 
@@ -218,25 +219,48 @@ MsmPlan msm_plan(uint32_t n) {
 
 // Enqueue the device part of an MSM on s; the window sums (W x 96 bytes of
 // halo2 words X, Y, Z) land at ws + off_wout.
-int msm_enqueue(const MsmPlan& p, const void* d_scalars, const void* d_bases, void* ws, hipStream_t s) {
+int msm_enqueue(Engine* eng, const MsmPlan& p, const void* d_scalars, const void* d_bases, void* ws,
+                hipStream_t s) {
   char* b = (char*)ws;
   const uint32_t nb = p.windows * p.half;
   uint32_t* counts = (uint32_t*)(b + p.off_counts);
-  hipError_t e = launch_reset_counters(counts, nb, s);
-  if (e == hipSuccess)
-    e = launch_msm_digits(d_scalars, d_bases, p.n, p.c, p.windows, b + p.off_blimbs, (int32_t*)(b + p.off_digits),
-                          counts, s);
-  if (e == hipSuccess)
-    e = launch_msm_scan(counts, nb, (uint32_t*)(b + p.off_sums), (uint32_t*)(b + p.off_offsets),
-                        (uint32_t*)(b + p.off_cursor), s);
-  if (e == hipSuccess)
-    e = launch_msm_scatter((const int32_t*)(b + p.off_digits), p.n, p.c, p.windows, (uint32_t*)(b + p.off_cursor),
-                           (uint32_t*)(b + p.off_list), s);
-  if (e == hipSuccess)
-    e = launch_msm_buckets(b + p.off_blimbs, (const uint32_t*)(b + p.off_offsets), counts,
-                           (const uint32_t*)(b + p.off_list), nb, b + p.off_buckets, s);
-  if (e == hipSuccess) e = launch_msm_chunks(b + p.off_buckets, p.half, p.run, p.runs, p.windows, b + p.off_parts, s);
-  if (e == hipSuccess) e = launch_msm_windows(b + p.off_parts, p.runs, p.windows, b + p.off_wout, s);
+  const bool timing = eng->pass_timing();
+  hipError_t e = hipSuccess;
+  // one launch (or launch group), bracketed by events while timing is on
+  auto step = [&](const char* name, auto&& launch) {
+    if (e != hipSuccess) return;
+    hipEvent_t ea = nullptr, eb = nullptr;
+    if (timing) {
+      eng->timed_events(&ea, &eb);
+      if (ea) (void)hipEventRecord(ea, s);
+    }
+    e = launch();
+    if (timing) {
+      if (eb) (void)hipEventRecord(eb, s);
+      eng->finish_timed(ea, eb, name);
+    }
+  };
+  step("reset_counters_kernel", [&] { return launch_reset_counters(counts, nb, s); });
+  step("msm_digits_kernel", [&] {
+    return launch_msm_digits(d_scalars, d_bases, p.n, p.c, p.windows, b + p.off_blimbs, (int32_t*)(b + p.off_digits),
+                             counts, s);
+  });
+  step("msm_scan (3 kernels)", [&] {
+    return launch_msm_scan(counts, nb, (uint32_t*)(b + p.off_sums), (uint32_t*)(b + p.off_offsets),
+                           (uint32_t*)(b + p.off_cursor), s);
+  });
+  step("msm_scatter_kernel", [&] {
+    return launch_msm_scatter((const int32_t*)(b + p.off_digits), p.n, p.c, p.windows, (uint32_t*)(b + p.off_cursor),
+                              (uint32_t*)(b + p.off_list), s);
+  });
+  step("msm_buckets_kernel", [&] {
+    return launch_msm_buckets(b + p.off_blimbs, (const uint32_t*)(b + p.off_offsets), counts,
+                              (const uint32_t*)(b + p.off_list), nb, b + p.off_buckets, s);
+  });
+  step("msm_chunks_kernel",
+       [&] { return launch_msm_chunks(b + p.off_buckets, p.half, p.run, p.runs, p.windows, b + p.off_parts, s); });
+  step("msm_windows_kernel",
+       [&] { return launch_msm_windows(b + p.off_parts, p.runs, p.windows, b + p.off_wout, s); });
   if (e != hipSuccess) {
     (void)hipGetLastError();
     return ZK_ERR_HIP;
@@ -284,7 +308,7 @@ int zkgpu_msm_device(int device_id, const void* d_scalars, const void* d_bases, 
   const MsmPlan p = msm_plan((uint32_t)length);
   WsLease lease = e->lease_workspace(s, p.bytes, WS_MSM, &st);
   if (st) return st;
-  st = msm_enqueue(p, d_scalars, d_bases, lease.ptr(), s);
+  st = msm_enqueue(e, p, d_scalars, d_bases, lease.ptr(), s);
   std::vector<uint64_t> wsum((size_t)p.windows * 12);
   if (st == ZK_OK &&
       hipMemcpyAsync(wsum.data(), (char*)lease.ptr() + p.off_wout, wsum.size() * 8, hipMemcpyDeviceToHost, s) !=
@@ -314,7 +338,7 @@ uint64_t parallel_msm_api(size_t device_id, const uint64_t* data, const void* ba
   char* io = (char*)ctx->io->ptr;
   st = stage_h2d((int)device_id, s, io, data, sbytes);
   if (st == ZK_OK) st = stage_h2d((int)device_id, s, io + sbytes, bases, bbytes);
-  if (st == ZK_OK) st = msm_enqueue(p, io, io + sbytes, ctx->aux->ptr, s);
+  if (st == ZK_OK) st = msm_enqueue(e, p, io, io + sbytes, ctx->aux->ptr, s);
   std::vector<uint64_t> wsum((size_t)p.windows * 12);
   if (st == ZK_OK &&
       hipMemcpyAsync(wsum.data(), (char*)ctx->aux->ptr + p.off_wout, wsum.size() * 8, hipMemcpyDeviceToHost, s) !=
@@ -324,6 +348,15 @@ uint64_t parallel_msm_api(size_t device_id, const uint64_t* data, const void* ba
   if (st) return (uint64_t)st;
   msm_combine(p, wsum.data(), (uint64_t*)acc);
   return ZK_OK;
+}
+
+int zkgpu_fill_random_points(int device_id, void* d_out, uint64_t count, uint64_t seed, uint64_t start,
+                             void* stream) {
+  if (!d_out && count) return ZK_ERR_INVALID;
+  int st = ZK_OK;
+  if (!Engine::get(device_id, &st)) return st;
+  if (count == 0) return ZK_OK;
+  return launch_fill_random_points(d_out, count, seed, start, (hipStream_t)stream) == hipSuccess ? ZK_OK : ZK_ERR_HIP;
 }
 
 uint64_t zkgpu_msm_workspace_bytes(uint64_t length, uint32_t* window_bits) {

@@ -286,7 +286,54 @@ __global__ void __launch_bounds__(256) msm_windows_kernel(const PointLimbs* __re
   }
 }
 
+// Synthetic bases: point i = k_i G for the generator G = (1, 2), k_i a
+// nonzero 64-bit integer from SplitMix64 of (seed, start + i); affine, halo2
+// Fq Montgomery words (x then y).
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__global__ void fill_random_points_kernel(uint4* __restrict__ out, uint64_t count, uint64_t seed, uint64_t start) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count) return;
+  const uint64_t k = mix64(seed * 0xD1B54A32D192ED03ull + (start + t) * 0x9E3779B97F4A7C15ull) | 1ull;
+  // G = (1, 2) in the 2^261 domain
+  g1a g;
+  g.x = fq::q_from_const(fq::ONE261);
+  g.y = fq::q_dbl(g.x);
+  g.inf = false;
+  g1j acc = g1_identity();
+  for (int bit = 63 - __clzll(k); bit >= 0; --bit) {
+    acc = g1_double(acc);
+    if ((k >> bit) & 1ull) acc = g1_add_mixed(acc, g);
+  }
+  // affine: z^-1 = z^(q-2)
+  fe zi = fq::q_from_const(fq::ONE261);
+  for (int w = 7; w >= 0; --w) {
+    const uint32_t e = fq::p32(w) - (w == 0 ? 2u : 0u);
+    for (int bit = 31; bit >= 0; --bit) {
+      zi = fq::q_sqr(zi);
+      if ((e >> bit) & 1u) zi = fq::q_mul(zi, acc.z);
+    }
+  }
+  const fe zi2 = fq::q_sqr(zi), zi3 = fq::q_mul(zi2, zi);
+  uint32_t x[8], y[8];
+  fq::q_to_words(x, fq::q_mul(acc.x, zi2));
+  fq::q_to_words(y, fq::q_mul(acc.y, zi3));
+  uint4* o = out + 4 * t;
+  o[0] = make_uint4(x[0], x[1], x[2], x[3]);
+  o[1] = make_uint4(x[4], x[5], x[6], x[7]);
+  o[2] = make_uint4(y[0], y[1], y[2], y[3]);
+  o[3] = make_uint4(y[4], y[5], y[6], y[7]);
+}
+
 // ---------------------------------------------------------------- launchers
+
+hipError_t launch_fill_random_points(void* out, uint64_t count, uint64_t seed, uint64_t start, hipStream_t s) {
+  fill_random_points_kernel<<<(uint32_t)((count + 63) / 64), 64, 0, s>>>((uint4*)out, count, seed, start);
+  return hipGetLastError();
+}
 
 hipError_t launch_msm_digits(const void* scalars, const void* bases, uint32_t n, uint32_t c, uint32_t windows,
                              void* blimbs, int32_t* digits, uint32_t* counts, hipStream_t s) {
