@@ -19,11 +19,10 @@
 namespace zkgpu {
 
 hipError_t launch_msm_digits(const void* scalars, const void* bases, uint32_t n, uint32_t c, uint32_t windows,
-                             void* blimbs, int32_t* digits, uint32_t* counts, hipStream_t s);
-hipError_t launch_msm_scan(const uint32_t* counts, uint32_t m, uint32_t* sums, uint32_t* offsets, uint32_t* cursor,
-                           hipStream_t s);
-hipError_t launch_msm_scatter(const int32_t* digits, uint32_t n, uint32_t c, uint32_t windows, uint32_t* cursor,
-                              uint32_t* list, hipStream_t s);
+                             void* blimbs, int32_t* digits, uint32_t* ranks, uint32_t* counts, hipStream_t s);
+hipError_t launch_msm_scan(const uint32_t* counts, uint32_t m, uint32_t* sums, uint32_t* offsets, hipStream_t s);
+hipError_t launch_msm_scatter(const int32_t* digits, const uint32_t* ranks, uint32_t n, uint32_t c, uint32_t windows,
+                              const uint32_t* offsets, uint32_t* list, hipStream_t s);
 hipError_t launch_msm_buckets(const void* blimbs, const uint32_t* offsets, const uint32_t* counts,
                               const uint32_t* list, uint32_t nbuckets, void* buckets, hipStream_t s);
 hipError_t launch_msm_chunks(const void* buckets, uint32_t half, uint32_t run, uint32_t runs_per_window,
@@ -179,7 +178,7 @@ Jac j_add(const Jac& p, const Jac& q) {  // add-2007-bl
 // Window size and workspace of an n-point MSM.
 struct MsmPlan {
   uint32_t n = 0, c = 0, windows = 0, half = 0, run = 0, runs = 0;
-  size_t off_blimbs = 0, off_digits = 0, off_list = 0, off_counts = 0, off_offsets = 0, off_cursor = 0,
+  size_t off_blimbs = 0, off_digits = 0, off_list = 0, off_counts = 0, off_offsets = 0, off_ranks = 0,
          off_sums = 0, off_buckets = 0, off_parts = 0, off_wout = 0, bytes = 0;
 };
 
@@ -208,7 +207,7 @@ MsmPlan msm_plan(uint32_t n) {
   p.off_list = take((size_t)p.windows * n * 4);
   p.off_counts = take(nb * 4);
   p.off_offsets = take(nb * 4);
-  p.off_cursor = take(nb * 4);
+  p.off_ranks = take((size_t)p.windows * n * 4);
   p.off_sums = take(((nb + 1023) / 1024) * 4);
   p.off_buckets = take(nb * msm_point_limb_bytes());
   p.off_parts = take((size_t)p.windows * p.runs * msm_point_limb_bytes());
@@ -243,15 +242,14 @@ int msm_enqueue(Engine* eng, const MsmPlan& p, const void* d_scalars, const void
   step("reset_counters_kernel", [&] { return launch_reset_counters(counts, nb, s); });
   step("msm_digits_kernel", [&] {
     return launch_msm_digits(d_scalars, d_bases, p.n, p.c, p.windows, b + p.off_blimbs, (int32_t*)(b + p.off_digits),
-                             counts, s);
+                             (uint32_t*)(b + p.off_ranks), counts, s);
   });
   step("msm_scan (3 kernels)", [&] {
-    return launch_msm_scan(counts, nb, (uint32_t*)(b + p.off_sums), (uint32_t*)(b + p.off_offsets),
-                           (uint32_t*)(b + p.off_cursor), s);
+    return launch_msm_scan(counts, nb, (uint32_t*)(b + p.off_sums), (uint32_t*)(b + p.off_offsets), s);
   });
   step("msm_scatter_kernel", [&] {
-    return launch_msm_scatter((const int32_t*)(b + p.off_digits), p.n, p.c, p.windows, (uint32_t*)(b + p.off_cursor),
-                              (uint32_t*)(b + p.off_list), s);
+    return launch_msm_scatter((const int32_t*)(b + p.off_digits), (const uint32_t*)(b + p.off_ranks), p.n, p.c,
+                              p.windows, (const uint32_t*)(b + p.off_offsets), (uint32_t*)(b + p.off_list), s);
   });
   step("msm_buckets_kernel", [&] {
     return launch_msm_buckets(b + p.off_blimbs, (const uint32_t*)(b + p.off_offsets), counts,

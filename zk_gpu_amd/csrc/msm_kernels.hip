@@ -9,10 +9,11 @@
 // all points are shared:
 //
 //   msm_digits    one thread per point: base to limb form, scalar to signed
-//                 digits, bucket histogram (atomics)
+//                 digits, bucket histogram (one atomic per digit, whose old
+//                 value is the point's rank in its bucket)
 //   scan          exclusive prefix sum of the bucket counts (three launches)
 //   msm_scatter   one thread per point: its index (sign in bit 31) into each
-//                 bucket's list
+//                 bucket's list, at the bucket's offset plus its rank
 //   msm_buckets   one thread per bucket: sum of its points (mixed additions)
 //   msm_chunks    one thread per run of L buckets of a window: running sums
 //                 give sum_j (j+1) B_j over the run
@@ -88,10 +89,12 @@ __device__ __forceinline__ fe fr_from_mont_factor() {
 }
 
 // Per point: base in limb form; signed digits of the scalar; bucket counts.
-// digits[w * n + i] is the digit of window w (0 for identity bases).
+// digits[w * n + i] is the digit of window w (0 for identity bases),
+// ranks[w * n + i] the point's rank among that bucket's points.
 __global__ void msm_digits_kernel(const uint4* __restrict__ scalars, const uint4* __restrict__ bases, uint32_t n,
                                   uint32_t c, uint32_t windows, BaseLimbs* __restrict__ blimbs,
-                                  int32_t* __restrict__ digits, uint32_t* __restrict__ counts) {
+                                  int32_t* __restrict__ digits, uint32_t* __restrict__ ranks,
+                                  uint32_t* __restrict__ counts) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint4 bx0 = bases[4 * (size_t)i], bx1 = bases[4 * (size_t)i + 1];
@@ -127,7 +130,7 @@ __global__ void msm_digits_kernel(const uint4* __restrict__ scalars, const uint4
     }
     if (inf) d = 0;
     digits[(size_t)w * n + i] = d;
-    if (d != 0) atomicAdd(&counts[w * half + (uint32_t)(d < 0 ? -d : d) - 1u], 1u);
+    if (d != 0) ranks[(size_t)w * n + i] = atomicAdd(&counts[w * half + (uint32_t)(d < 0 ? -d : d) - 1u], 1u);
   }
 }
 
@@ -174,7 +177,7 @@ __global__ void scan_sums_kernel(uint32_t* sums, uint32_t nb) {
   }
 }
 __global__ void scan_apply_kernel(const uint32_t* __restrict__ in, uint32_t m, const uint32_t* __restrict__ sums,
-                                  uint32_t* __restrict__ out, uint32_t* __restrict__ cursor) {
+                                  uint32_t* __restrict__ out) {
   __shared__ uint32_t vals[SCAN_BLOCK];
   const uint32_t base = blockIdx.x * SCAN_BLOCK;
   for (uint32_t k = threadIdx.x; k < SCAN_BLOCK; k += 256) vals[k] = base + k < m ? in[base + k] : 0;
@@ -189,17 +192,15 @@ __global__ void scan_apply_kernel(const uint32_t* __restrict__ in, uint32_t m, c
   }
   __syncthreads();
   for (uint32_t k = threadIdx.x; k < SCAN_BLOCK; k += 256) {
-    if (base + k < m) {
-      out[base + k] = vals[k];
-      cursor[base + k] = vals[k];
-    }
+    if (base + k < m) out[base + k] = vals[k];
   }
 }
 
 // Each point's index (bit 31: negative digit) into the list of every bucket
 // it falls in.
-__global__ void msm_scatter_kernel(const int32_t* __restrict__ digits, uint32_t n, uint32_t c, uint32_t windows,
-                                   uint32_t* __restrict__ cursor, uint32_t* __restrict__ list) {
+__global__ void msm_scatter_kernel(const int32_t* __restrict__ digits, const uint32_t* __restrict__ ranks, uint32_t n,
+                                   uint32_t c, uint32_t windows, const uint32_t* __restrict__ offsets,
+                                   uint32_t* __restrict__ list) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t half = 1u << (c - 1);
@@ -207,8 +208,7 @@ __global__ void msm_scatter_kernel(const int32_t* __restrict__ digits, uint32_t 
     const int32_t d = digits[(size_t)w * n + i];
     if (d == 0) continue;
     const uint32_t key = w * half + (uint32_t)(d < 0 ? -d : d) - 1u;
-    const uint32_t pos = atomicAdd(&cursor[key], 1u);
-    list[pos] = i | (d < 0 ? 0x80000000u : 0u);
+    list[offsets[key] + ranks[(size_t)w * n + i]] = i | (d < 0 ? 0x80000000u : 0u);
   }
 }
 
@@ -336,22 +336,21 @@ hipError_t launch_fill_random_points(void* out, uint64_t count, uint64_t seed, u
 }
 
 hipError_t launch_msm_digits(const void* scalars, const void* bases, uint32_t n, uint32_t c, uint32_t windows,
-                             void* blimbs, int32_t* digits, uint32_t* counts, hipStream_t s) {
+                             void* blimbs, int32_t* digits, uint32_t* ranks, uint32_t* counts, hipStream_t s) {
   msm_digits_kernel<<<(n + 255) / 256, 256, 0, s>>>((const uint4*)scalars, (const uint4*)bases, n, c, windows,
-                                                    (BaseLimbs*)blimbs, digits, counts);
+                                                    (BaseLimbs*)blimbs, digits, ranks, counts);
   return hipGetLastError();
 }
-hipError_t launch_msm_scan(const uint32_t* counts, uint32_t m, uint32_t* sums, uint32_t* offsets, uint32_t* cursor,
-                           hipStream_t s) {
+hipError_t launch_msm_scan(const uint32_t* counts, uint32_t m, uint32_t* sums, uint32_t* offsets, hipStream_t s) {
   const uint32_t nb = (m + SCAN_BLOCK - 1) / SCAN_BLOCK;
   scan_block_sums_kernel<<<nb, 256, 0, s>>>(counts, m, sums);
   scan_sums_kernel<<<1, 256, 0, s>>>(sums, nb);
-  scan_apply_kernel<<<nb, 256, 0, s>>>(counts, m, sums, offsets, cursor);
+  scan_apply_kernel<<<nb, 256, 0, s>>>(counts, m, sums, offsets);
   return hipGetLastError();
 }
-hipError_t launch_msm_scatter(const int32_t* digits, uint32_t n, uint32_t c, uint32_t windows, uint32_t* cursor,
-                              uint32_t* list, hipStream_t s) {
-  msm_scatter_kernel<<<(n + 255) / 256, 256, 0, s>>>(digits, n, c, windows, cursor, list);
+hipError_t launch_msm_scatter(const int32_t* digits, const uint32_t* ranks, uint32_t n, uint32_t c, uint32_t windows,
+                              const uint32_t* offsets, uint32_t* list, hipStream_t s) {
+  msm_scatter_kernel<<<(n + 255) / 256, 256, 0, s>>>(digits, ranks, n, c, windows, offsets, list);
   return hipGetLastError();
 }
 hipError_t launch_msm_buckets(const void* blimbs, const uint32_t* offsets, const uint32_t* counts,
