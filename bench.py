@@ -515,6 +515,101 @@ def bench_serving(torch, Z, dev, stream, count=10_000, loads=(0.5, 0.9, 1.5), se
     return res
 
 
+def _host_call(Z, buf, k, op):
+    """One drop-in host-buffer call (the reference's per-call path: copy in,
+    transform, copy out), in place on buf."""
+    n = 1 << k
+    inverse = op in (1, 3)
+    w = Z.domain_constant(k, 1 if inverse else 0)
+    if op == 0:
+        rc = Z.parallel_fft_api(0, buf, w, None, n, k, 0)
+    elif op == 1:
+        rc = Z.parallel_ifft_api(0, buf, w, None, n, k, 0, Z.domain_constant(k, 2))
+    elif op == 2:
+        rc = Z.parallel_coeff_to_extended_part_api(0, buf, w, None, n, k, 0, Z.domain_constant(k, 3))
+    else:
+        rc = Z.zkgpu_coset_ifft_api(0, buf, w, n, k, Z.domain_constant(k, 2), Z.domain_constant(k, 4))
+    Z.check(rc, "host call")
+
+
+def bench_host_modes(torch, Z, dev, stream, serve_count=1000, slots=24):
+    """Host-buffer modes (data in pageable host memory before and after, as
+    the reference's FFI takes it): the plain path is one drop-in call per job
+    (parallel_*_api: copy in, transform, copy out); the served path submits the
+    same jobs to the server as ZKGPU_JOB_HOST_BUFFERS jobs, which overlaps one
+    job's upload with others' passes and downloads. Every job transforms its
+    host buffer in place (a ring of `slots` buffers, at most `slots` jobs
+    outstanding). Two workloads: the 64-polynomial batch mix (k 12..20) and
+    `serve_count` jobs of the serving mix (k 14..22)."""
+    import random
+    import threading
+
+    import numpy as np
+    src = torch.empty((1 << 22) * 4, dtype=torch.int64, device=dev)
+    Z.fill_random(src.data_ptr(), 1 << 22, 9090, stream=stream.cuda_stream)
+    torch.cuda.synchronize()
+    host_src = src.cpu().numpy().view(np.uint64).reshape(-1, 4)
+    ring = [np.empty(((1 << 22), 4), dtype=np.uint64) for _ in range(slots)]
+    for r in ring:
+        np.copyto(r, host_src)  # touch every page once
+    for k in range(12, 23):
+        for op in range(4):
+            Z.prepare_std(k, op)
+
+    def run_plain(spec):
+        t0 = time.perf_counter()
+        for i, (k, op) in enumerate(spec):
+            _host_call(Z, ring[i % slots][:1 << k], k, op)
+        return time.perf_counter() - t0
+
+    def run_served(spec):
+        srv = Z.Server(budget_bytes=64 << 30)
+        sem = threading.Semaphore(slots)
+        tickets = [None] * len(spec)
+        lat = []
+
+        def waiter():
+            for i in range(len(spec)):
+                while tickets[i] is None:
+                    time.sleep(0.00005)
+                r = srv.wait(tickets[i])
+                lat.append(r.t_done_ns - r.t_submit_ns)
+                srv.release(tickets[i])
+                sem.release()
+
+        th = threading.Thread(target=waiter)
+        t0 = time.perf_counter()
+        th.start()
+        for i, (k, op) in enumerate(spec):
+            sem.acquire()
+            buf = ring[i % slots]
+            tickets[i] = srv.submit(Z.make_job(buf.ctypes.data, buf.ctypes.data, k, op, host=True))
+        th.join()
+        el = time.perf_counter() - t0
+        srv.close()
+        return el, lat
+
+    out = {}
+    rnd = random.Random(4)
+    batch_spec = [(rnd.randint(12, 20), rnd.choice([0, 1, 2, 3])) for _ in range(64)]
+    serve_spec = _serve_spec(serve_count, seed=6)
+    for name, spec, reps in (("batch_mix_64", batch_spec, 3), (f"serving_mix_{serve_count}", serve_spec, 1)):
+        elements = sum(1 << k for k, _ in spec)
+        run_plain(spec[:8])
+        run_served(spec[:8])
+        plain_s = min(run_plain(spec) for _ in range(reps))
+        served = [run_served(spec) for _ in range(reps)]
+        served_s, lat = min(served, key=lambda x: x[0])
+        out[name] = {"jobs": len(spec), "elements": elements, "plain_s": plain_s,
+                     "plain_elem_per_s": elements / plain_s, "served_s": served_s,
+                     "served_elem_per_s": elements / served_s, "speedup_vs_plain": plain_s / served_s,
+                     "served_p50_ms": _pct(lat, 0.5) / 1e6, "served_p99_ms": _pct(lat, 0.99) / 1e6,
+                     "outstanding_jobs": slots}
+    out["note"] = ("host-buffer mode: inputs and results in pageable host memory; plain = one parallel_*_api call "
+                   "per job (copy in, transform, copy out); served = ZKGPU_JOB_HOST_BUFFERS jobs, in place")
+    return out
+
+
 def _device_copy(torch, d_ptr, k):
     """Tensor copy of a server-owned result buffer."""
     import ctypes
@@ -700,6 +795,7 @@ def main():
 
     if not args.no_host:
         res["host_api_k20"] = bench_host_api(Z)
+        res["host_buffer_modes"] = bench_host_modes(torch, Z, dev, stream)
     if not args.no_parts:
         res["extended_parts"] = bench_parts(torch, Z, dev, stream)
 

@@ -174,7 +174,8 @@ int zkgpu_plan_set_kind(uint32_t log_n, int kind, uint32_t npass, const uint32_t
  * same job run alone.
  * --------------------------------------------------------------------- */
 
-#define ZKGPU_JOB_STD_DOMAIN 1u /* ignore omega/divisor/coset: standard domain of size 2^log_n, g = 7 */
+#define ZKGPU_JOB_STD_DOMAIN 1u   /* ignore omega/divisor/coset: standard domain of size 2^log_n, g = 7 */
+#define ZKGPU_JOB_HOST_BUFFERS 2u /* server jobs only: d_in / d_out are host buffers (see zkgpu_server_submit) */
 
 typedef struct zkgpu_job {
   const void* d_in;     /* device pointer, 2^log_n elements */
@@ -261,9 +262,16 @@ int zkgpu_server_create(int device_id, uint64_t budget_bytes, uint32_t max_activ
                         uint64_t* handle);
 /* Queue a job (non-blocking). job->d_in must hold its data when the job is
  * submitted (the work producing it complete: the server runs on its own
- * stream) and stay valid and unmodified until the job is done; job->d_out = NULL asks for a server-owned result buffer
- * (valid until zkgpu_server_release). A job whose footprint alone exceeds the
- * budget returns ZKGPU_ERR_BUDGET. */
+ * stream) and stay valid and unmodified until the job is done; job->d_out =
+ * NULL asks for a server-owned result buffer (valid until
+ * zkgpu_server_release). With ZKGPU_JOB_HOST_BUFFERS, d_in and d_out are host
+ * buffers (pageable memory is fine; they may be equal): the server takes a
+ * device buffer for the job at admission, copies the input in on an upload
+ * stream, runs the passes and copies the result to d_out on a download
+ * stream; the job is done once d_out holds it, and d_in must stay unmodified
+ * until then. Uploads, other jobs' passes and downloads overlap. A host job's
+ * footprint is its 32-byte-per-element device buffer plus the workspace. A
+ * job whose footprint alone exceeds the budget returns ZKGPU_ERR_BUDGET. */
 int zkgpu_server_submit(uint64_t handle, const zkgpu_job* job, uint64_t* ticket);
 /* State of a job; wait_ms < 0 waits for completion, 0 polls, > 0 waits at
  * most that long. */

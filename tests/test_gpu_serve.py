@@ -239,3 +239,68 @@ def test_serve_c5_poisson_stream(torch_cuda):
             bad.append((i, k, op))
     assert not bad, bad
     assert sum(1 for i in keep if spec[i][0] == 22) == 4
+
+
+def test_serve_host_buffers(torch_cuda):
+    """Jobs on host buffers (ZKGPU_JOB_HOST_BUFFERS): the server stages inputs
+    in and results out on its own streams while other jobs run. Every result
+    equals the oracle (k <= 14) or the drop-in host entry point, in place and
+    out of place, mixed with device jobs in one stream of submissions."""
+    torch = torch_cuda
+    rnd = random.Random(31)
+    specs = [(rnd.randint(9, 20), rnd.choice(OPNAMES)) for _ in range(60)]
+    xs = [O.gen(71_000 + i, 1 << k) for i, (k, _) in enumerate(specs)]
+    outs = [x if i % 4 == 0 else np.empty_like(x) for i, x in enumerate(xs)]  # every fourth in place
+    keep_in = [x.copy() for x in xs]
+    dev_x = dev_empty(torch, 1 << 16)
+    Z.fill_random(ptr(dev_x), 1 << 16, 71_999)
+    dev_out = dev_empty(torch, 1 << 16)
+    torch.cuda.synchronize()
+    srv = Z.Server()
+    try:
+        tickets = []
+        for i, ((k, op), x, o) in enumerate(zip(specs, xs, outs)):
+            tickets.append(srv.submit(Z.make_job(x, o, k, Z.OPS[op], host=True)))
+            if i == 30:
+                td = srv.submit(Z.make_job(ptr(dev_x), ptr(dev_out), 16, Z.OP_FWD))
+        res = [srv.wait(t) for t in tickets]
+        srv.wait(td)
+        st = srv.stats()
+    finally:
+        srv.close()
+    assert st["failed"] == 0 and st["completed"] == len(specs) + 1 and st["in_use_bytes"] == 0
+    assert any(r.iter_jobs_max >= 2 for r in res)
+    for (k, op), x0, o in zip(specs, keep_in, outs):
+        if k <= 14:
+            want = O.run_op(op, x0, k, threads=8)
+        else:
+            want = x0.copy()
+            inverse = op in ("inv", "coset_inv")
+            w = Z.domain_constant(k, 1 if inverse else 0)
+            n = 1 << k
+            if op == "fwd":
+                Z.check(Z.parallel_fft_api(0, want, w, None, n, k, 0))
+            elif op == "inv":
+                Z.check(Z.parallel_ifft_api(0, want, w, None, n, k, 0, Z.domain_constant(k, 2)))
+            elif op == "coset_fwd":
+                Z.check(Z.parallel_coeff_to_extended_part_api(0, want, w, None, n, k, 0, Z.domain_constant(k, 3)))
+            else:
+                Z.check(Z.zkgpu_coset_ifft_api(0, want, w, n, k, Z.domain_constant(k, 2), Z.domain_constant(k, 4)))
+        assert (o == want).all(), (k, op)
+    assert (to_host(dev_out) == _alone(torch, dev_x, 16, "fwd")).all()
+
+
+def test_serve_host_buffers_destroy_in_flight(torch_cuda):
+    """Destroying a server while host jobs are uploading, running and
+    downloading finishes or cancels them without hanging."""
+    xs = [O.gen(72_000 + i, 1 << 18) for i in range(12)]
+    srv = Z.Server()
+    for x in xs:
+        srv.submit(Z.make_job(x, x, 18, Z.OP_FWD, host=True))
+    srv.close()
+
+
+def test_batch_rejects_host_buffers(torch_cuda):
+    x = O.gen(1, 1 << 10)
+    with pytest.raises(Z.ZkGpuError):
+        Z.Batch([Z.make_job(x, x, 10, Z.OP_FWD, host=True)])

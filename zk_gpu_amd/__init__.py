@@ -277,12 +277,21 @@ class Job(ctypes.Structure):
                 ("omega", ctypes.c_uint64 * 4), ("divisor", ctypes.c_uint64 * 4), ("coset", ctypes.c_uint64 * 4)]
 
 
-def make_job(d_in, d_out, log_n, op, omega=None, divisor=None, c=None):
-    """A job on device pointers; without constants it uses the standard domain (g = 7)."""
+JOB_HOST_BUFFERS = 2
+
+
+def make_job(d_in, d_out, log_n, op, omega=None, divisor=None, c=None, host=False):
+    """A job on device pointers (host=True: host buffers, server jobs only,
+    pointers as ints or uint64 numpy arrays); without constants it uses the
+    standard domain (g = 7)."""
     j = Job()
+    if host:
+        d_in = d_in if isinstance(d_in, int) else d_in.ctypes.data
+        d_out = d_out if isinstance(d_out, int) else d_out.ctypes.data
     j.d_in, j.d_out, j.log_n, j.op = d_in, d_out, log_n, op
+    j.flags = JOB_HOST_BUFFERS if host else 0
     if omega is None:
-        j.flags = JOB_STD_DOMAIN
+        j.flags |= JOB_STD_DOMAIN
     else:
         for name, v in (("omega", omega), ("divisor", divisor), ("coset", c)):
             if v is not None:

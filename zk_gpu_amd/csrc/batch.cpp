@@ -85,6 +85,7 @@ int batch_build(int device, const zkgpu_job* cjobs, uint32_t count, Batch** out,
   b->jobs.resize(count);
   b->tables.resize(count);
   for (uint32_t i = 0; i < count; ++i) {
+    if (cjobs[i].flags & ZKGPU_JOB_HOST_BUFFERS) return ZK_ERR_INVALID;  // batches take device buffers
     st = job_from_c(cjobs[i], b->jobs[i], false);
     if (st) return st;
     st = e->prepare(b->jobs[i], b->tables[i], nullptr, PLAN_BATCH);

@@ -26,12 +26,14 @@ constexpr size_t kChunk = 8u << 20;        // bytes per pinned buffer
 constexpr size_t kDirectBelow = 1u << 20;  // smaller copies go straight through hipMemcpyAsync
 constexpr size_t kSlice = 1u << 20;        // host-copy work unit
 
-// Fixed pool of host threads for the copies into and out of pinned memory.
+// Fixed pools of host threads for the copies into and out of pinned memory:
+// one for uploads, one for downloads, so that a download's host copy does not
+// queue behind an upload's (the server runs both at once).
 class CopyPool {
  public:
-  static CopyPool& get() {
-    static CopyPool p;
-    return p;
+  static CopyPool& get(int direction) {
+    static CopyPool up, down;
+    return direction ? down : up;
   }
   // dst[i] = src[i] for bytes, split over the pool; returns when done. One
   // copy at a time: concurrent callers queue here.
@@ -60,7 +62,7 @@ class CopyPool {
  private:
   CopyPool() {
     unsigned hc = std::thread::hardware_concurrency();
-    const unsigned nw = std::min(7u, hc > 2 ? hc / 2 : 1u);
+    const unsigned nw = std::min(6u, hc > 4 ? hc / 2 - 1 : 1u);
     for (unsigned i = 0; i < nw; ++i) workers_.emplace_back([this] { loop(); });
   }
   ~CopyPool() {
@@ -149,7 +151,7 @@ int stage_h2d(int device, hipStream_t s, void* dst, const void* src, size_t byte
     const size_t off = i * kChunk, len = std::min(kChunk, bytes - off);
     // the buffer's previous DMA (chunk i-2, or an earlier copy on s) has left
     if (hipEventSynchronize(st->ev[b]) != hipSuccess) return ZK_ERR_HIP;
-    CopyPool::get().copy(st->pin[b], (const char*)src + off, len);
+    CopyPool::get(0).copy(st->pin[b], (const char*)src + off, len);
     if (hipMemcpyAsync((char*)dst + off, st->pin[b], len, hipMemcpyHostToDevice, s) != hipSuccess) return ZK_ERR_HIP;
     if (hipEventRecord(st->ev[b], s) != hipSuccess) return ZK_ERR_HIP;
   }
@@ -175,7 +177,7 @@ int stage_d2h(int device, hipStream_t s, void* dst, const void* src, size_t byte
     const int b = (int)(i & 1);
     const size_t off = i * kChunk, len = std::min(kChunk, bytes - off);
     if (hipEventSynchronize(st->ev[b]) != hipSuccess) return ZK_ERR_HIP;
-    CopyPool::get().copy((char*)dst + off, st->pin[b], len);
+    CopyPool::get(1).copy((char*)dst + off, st->pin[b], len);
     if (i + 2 < n && !enqueue(i + 2)) return ZK_ERR_HIP;
   }
   return ZK_OK;

@@ -14,6 +14,15 @@
 //             admitted job (passes of one job go to successive iterations on
 //             one stream, so stream order is their dependency).
 //
+// Jobs on host buffers (ZKGPU_JOB_HOST_BUFFERS) get a device buffer from the
+// pool at admission; an uploader thread stages their input to it on its own
+// stream, a downloader thread stages the result back once the job's last
+// iteration has retired, so one job's upload, other jobs' passes and a third
+// job's download overlap (the reference copies in, transforms and copies out
+// one call at a time, cuda/fft.cu:284-285,388). A host job joins iterations
+// once its upload is enqueued; the iteration's stream waits for the upload's
+// event.
+//
 // Up to kDepth iterations are in flight, so the device never waits for the
 // host between iterations; a job submitted meanwhile joins at the next
 // iteration issued. Per-iteration descriptors go through pinned staging (one
@@ -120,6 +129,11 @@ struct SJob {
   size_t footprint = 0;
   int status = ZK_OK;
   bool done = false;
+  bool host = false;           // d_in / d_out are host buffers
+  bool ready = true;           // input on the device (host jobs: upload enqueued)
+  void* dev_io = nullptr;      // host jobs: device buffer (input, then output in place)
+  size_t io_bytes = 0;
+  hipEvent_t ev_up = nullptr;  // host jobs: the upload's completion on the upload stream
   uint32_t iters = 0, iter_jobs_max = 0;  // iterations run, most jobs in one of them
   uint64_t t_submit = 0, t_admit = 0, t_start = 0, t_done = 0;
 };
@@ -140,6 +154,8 @@ class Server {
 
   int init() {
     if (hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking) != hipSuccess) return ZK_ERR_HIP;
+    if (hipStreamCreateWithFlags(&up_stream_, hipStreamNonBlocking) != hipSuccess) return ZK_ERR_HIP;
+    if (hipStreamCreateWithFlags(&down_stream_, hipStreamNonBlocking) != hipSuccess) return ZK_ERR_HIP;
     int cus = 256;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_);
     grid_cap_ = (uint32_t)cus * persistent_wgs_per_cu();  // resident workgroups of the persistent kernel
@@ -154,6 +170,8 @@ class Server {
       memset(it.host, 0, stage_bytes_);
     }
     thread_ = std::thread([this] { loop(); });
+    uploader_ = std::thread([this] { io_loop(true); });
+    downloader_ = std::thread([this] { io_loop(false); });
     return ZK_OK;
   }
 
@@ -167,10 +185,21 @@ class Server {
       cv_work_.notify_all();
       thread_.join();
     }
+    // the scheduler has retired every iteration; the I/O threads finish
+    // their queues (the downloads of the last host jobs) and stop
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      io_stop_ = true;
+    }
+    cv_io_.notify_all();
+    if (uploader_.joinable()) uploader_.join();
+    if (downloader_.joinable()) downloader_.join();
     if (stream_) (void)hipStreamSynchronize(stream_);
     for (auto& kv : jobs_) {
       pool_.put(kv.second->out_owned, kv.second->out_bytes);
       pool_.put(kv.second->ws, kv.second->ws_bytes);
+      pool_.put(kv.second->dev_io, kv.second->io_bytes);
+      if (kv.second->ev_up) (void)hipEventDestroy(kv.second->ev_up);
     }
     pool_.trim();
     for (auto& it : ring_) {
@@ -179,6 +208,8 @@ class Server {
       if (it.dev) (void)hipFree(it.dev);
     }
     if (stream_) (void)hipStreamDestroy(stream_);
+    if (up_stream_) (void)hipStreamDestroy(up_stream_);
+    if (down_stream_) (void)hipStreamDestroy(down_stream_);
   }
 
   int submit(const zkgpu_job& cj, uint64_t* ticket) {
@@ -190,8 +221,14 @@ class Server {
     const Plan p = plan_for(k, PLAN_BATCH);
     j->npass = p.npass;
     j->ws_bytes = p.npass > 1 ? round256((size_t)36 << k) : 0;
-    j->out_bytes = cj.d_out ? 0 : round256((size_t)32 << k);
-    j->footprint = j->ws_bytes + j->out_bytes;
+    j->host = (cj.flags & ZKGPU_JOB_HOST_BUFFERS) != 0;
+    if (j->host) {
+      if (!cj.d_out) return ZK_ERR_INVALID;  // a host job's result goes to the caller's buffer
+      j->io_bytes = round256((size_t)32 << k);
+      j->ready = false;
+    }
+    j->out_bytes = (cj.d_out || j->host) ? 0 : round256((size_t)32 << k);
+    j->footprint = j->ws_bytes + j->out_bytes + j->io_bytes;
     if (j->footprint > budget_) return ZK_ERR_BUDGET;
     j->t_submit = now_ns();
     {
@@ -255,7 +292,7 @@ class Server {
     s->completed = completed_;
     s->failed = failed_;
     s->pending = pending_.size();
-    s->active = active_.size() + finishing_count_;
+    s->active = active_.size() + finishing_count_ + downloading_;
     s->iterations = iterations_;
     s->budget_bytes = budget_;
     s->in_use_bytes = in_use_;
@@ -276,7 +313,7 @@ class Server {
   // table preparation happens here (fresh tables are built on the
   // scheduler's own table stream and synchronised there, never on stream_).
   void admit() {
-    while (!pending_.empty() && active_.size() + finishing_count_ < max_active_) {
+    while (!pending_.empty() && active_.size() + finishing_count_ + downloading_ < max_active_) {
       SJob* j = pending_.front();
       if (in_use_ + j->footprint > budget_) break;  // arrival order: no overtaking
       pending_.pop_front();
@@ -291,6 +328,17 @@ class Server {
         if (!j->out_owned) st = ZK_ERR_NOMEM;
         j->spec.out = j->out_owned;
       }
+      if (st == ZK_OK && j->host) {
+        j->dev_io = pool_.get(j->io_bytes);
+        if (!j->dev_io) st = ZK_ERR_NOMEM;
+        if (st == ZK_OK && hipEventCreateWithFlags(&j->ev_up, hipEventDisableTiming) != hipSuccess) {
+          (void)hipGetLastError();
+          j->ev_up = nullptr;
+          st = ZK_ERR_HIP;
+        }
+        j->spec.in = j->dev_io;  // transformed in place on the device
+        j->spec.out = j->dev_io;
+      }
       uint32_t nd = 0;
       if (st == ZK_OK) st = eng_->build_descs(j->spec, j->tables, j->ws, j->descs, &nd);
       if (st != ZK_OK) {
@@ -298,6 +346,8 @@ class Server {
         j->ws = nullptr;
         pool_.put(j->out_owned, j->out_bytes);
         j->out_owned = nullptr;
+        pool_.put(j->dev_io, j->io_bytes);
+        j->dev_io = nullptr;
         fail(j, st);
         cv_done_.notify_all();
         continue;
@@ -306,6 +356,10 @@ class Server {
       in_use_ += j->footprint;
       peak_in_use_ = std::max(peak_in_use_, in_use_);
       active_.push_back(j);
+      if (j->host) {
+        up_q_.push_back(j);
+        cv_io_.notify_all();
+      }
     }
   }
 
@@ -313,7 +367,14 @@ class Server {
   int issue() {
     Iter& it = ring_[(head_ + inflight_) % kDepth];
     it.finishing.clear();
-    std::vector<SJob*> order(active_.begin(), active_.end());
+    std::vector<SJob*> order;
+    for (SJob* j : active_)
+      if (j->ready) order.push_back(j);
+    if (order.empty()) return ZK_OK;
+    hipError_t e = hipSuccess;
+    // host jobs starting now: this iteration waits for their uploads
+    for (SJob* j : order)
+      if (j->host && j->next == 0 && e == hipSuccess) e = hipStreamWaitEvent(stream_, j->ev_up, 0);
     // biggest transforms first so that the tile queue ends with short tiles
     std::stable_sort(order.begin(), order.end(),
                      [](const SJob* a, const SJob* b) { return a->cjob.log_n > b->cjob.log_n; });
@@ -342,12 +403,11 @@ class Server {
     active_.erase(std::remove_if(active_.begin(), active_.end(), [](SJob* j) { return j->next == j->npass; }),
                   active_.end());
     finishing_count_ += it.finishing.size();
-    hipError_t e = hipSuccess;
-    if (nd == 1) {
+    if (e == hipSuccess && nd == 1) {
       // one job in the iteration: its pass as a pass-kernel launch (the
       // descriptor travels as the kernel argument; no copy, no counter)
       e = launch_pass(hd[0], stream_);
-    } else {
+    } else if (e == hipSuccess) {
       e = hipMemcpyAsync(it.dev, it.host, desc_off + nd * sizeof(PassDesc), hipMemcpyHostToDevice, stream_);
       if (e == hipSuccess)
         e = launch_persistent((const PassDesc*)((char*)it.dev + desc_off), (const uint32_t*)((char*)it.dev + ts_off_),
@@ -362,6 +422,7 @@ class Server {
         pool_.put(j->ws, j->ws_bytes);
         j->ws = nullptr;
         in_use_ -= j->ws_bytes;
+        release_io(j);
         fail(j, ZK_ERR_HIP);
       }
       finishing_count_ -= it.finishing.size();
@@ -385,6 +446,14 @@ class Server {
         pool_.put(j->ws, j->ws_bytes);
         j->ws = nullptr;
         in_use_ -= j->ws_bytes;
+        if (q == hipSuccess && j->host) {
+          // the result goes back to the caller's buffer first
+          down_q_.push_back(j);
+          ++downloading_;
+          cv_io_.notify_all();
+          continue;
+        }
+        release_io(j);
         j->done = true;
         j->t_done = t;
         if (q == hipSuccess) {
@@ -404,15 +473,97 @@ class Server {
     }
   }
 
+  // A host job's device buffer back to the pool (mu_ held).
+  void release_io(SJob* j) {
+    if (!j->host) return;
+    if (j->dev_io) {
+      pool_.put(j->dev_io, j->io_bytes);
+      in_use_ -= j->io_bytes;
+      j->dev_io = nullptr;
+    }
+    if (j->ev_up) {
+      (void)hipEventDestroy(j->ev_up);
+      j->ev_up = nullptr;
+    }
+  }
+
   void fail_active(int st) {
+    std::vector<SJob*> keep;
     for (SJob* j : active_) {
+      if (j->host && !j->ready) {
+        keep.push_back(j);  // the uploader still owns it; it fails it when done
+        continue;
+      }
       pool_.put(j->ws, j->ws_bytes);
       j->ws = nullptr;
-      in_use_ -= j->footprint - j->out_bytes;
+      in_use_ -= j->ws_bytes;
+      release_io(j);
       fail(j, st);
     }
-    active_.clear();
+    active_.swap(keep);
+    broken_ = true;
     cv_done_.notify_all();
+  }
+
+  // Uploads (up = true) or downloads of host jobs, one job at a time, on the
+  // server's upload or download stream.
+  void io_loop(bool up) {
+    (void)hipSetDevice(device_);
+    hipStream_t s = up ? up_stream_ : down_stream_;
+    std::deque<SJob*>& q = up ? up_q_ : down_q_;
+    std::unique_lock<std::mutex> lk(mu_);
+    for (;;) {
+      cv_io_.wait(lk, [&] { return io_stop_ || !q.empty(); });
+      if (q.empty()) {
+        if (io_stop_) return;
+        continue;
+      }
+      SJob* j = q.front();
+      q.pop_front();
+      const size_t bytes = (size_t)32 << j->cjob.log_n;
+      lk.unlock();
+      int st;
+      if (up) {
+        st = stage_h2d(device_, s, j->dev_io, j->cjob.d_in, bytes);
+        if (st == ZK_OK && hipEventRecord(j->ev_up, s) != hipSuccess) st = ZK_ERR_HIP;
+        if (st) (void)hipStreamSynchronize(s);
+      } else {
+        st = stage_d2h(device_, s, j->cjob.d_out, j->dev_io, bytes);
+        if (st) (void)hipStreamSynchronize(s);
+      }
+      lk.lock();
+      if (up) {
+        if (st == ZK_OK && !broken_) {
+          j->ready = true;
+        } else {
+          active_.erase(std::remove(active_.begin(), active_.end(), j), active_.end());
+          pool_.put(j->ws, j->ws_bytes);
+          j->ws = nullptr;
+          in_use_ -= j->ws_bytes;
+          release_io(j);
+          fail(j, st ? st : ZK_ERR_HIP);
+          cv_done_.notify_all();
+        }
+        ++gen_;
+        submissions_.fetch_add(1, std::memory_order_release);  // wakes a waiting scheduler
+        cv_work_.notify_one();
+      } else {
+        release_io(j);
+        --downloading_;
+        j->done = true;
+        j->t_done = now_ns();
+        if (st == ZK_OK) {
+          ++completed_;
+          elements_done_ += 1ull << j->cjob.log_n;
+        } else {
+          j->status = st;
+          ++failed_;
+        }
+        ++gen_;
+        cv_done_.notify_all();
+        cv_work_.notify_one();  // memory came back
+      }
+    }
   }
 
   void loop() {
@@ -427,7 +578,9 @@ class Server {
       } else {
         admit();
       }
-      if (!active_.empty() && inflight_ < kDepth) {
+      const bool any_ready =
+          std::any_of(active_.begin(), active_.end(), [](const SJob* j) { return j->ready; });
+      if (any_ready && inflight_ < kDepth) {
         if (issue() != ZK_OK) {
           // the stream is unusable: wait for what was issued, then fail the rest
           lk.unlock();
@@ -472,6 +625,13 @@ class Server {
   const uint32_t max_active_;
   Pool pool_;
   hipStream_t stream_ = nullptr;
+  hipStream_t up_stream_ = nullptr, down_stream_ = nullptr;
+  std::thread uploader_, downloader_;
+  std::condition_variable cv_io_;
+  std::deque<SJob*> up_q_, down_q_;  // host jobs to upload / to download
+  size_t downloading_ = 0;
+  bool io_stop_ = false;
+  bool broken_ = false;  // the compute stream failed
   uint32_t grid_cap_ = 512;
   size_t ts_off_ = 0, stage_bytes_ = 0;
   Iter ring_[kDepth];
