@@ -508,6 +508,26 @@ def bench_serving(torch, Z, dev, stream, count=10_000, loads=(0.5, 0.9, 1.5), se
                              "plain_p50_ms": _pct(lat_p, 0.5) / 1e6, "plain_p99_ms": _pct(lat_p, 0.99) / 1e6,
                              "served_p50_ms": _pct(lat_s, 0.5) / 1e6, "served_p99_ms": _pct(lat_s, 0.99) / 1e6,
                              "served_elem_per_s": elements / s_s})
+    # per size class: plain per-call rate against the served rate of a stream
+    # of that size only (all jobs submitted at once)
+    per_k = []
+    for k in range(14, 23):
+        jobs = [(k, op % 4) for op in range(48)]
+        el = len(jobs) << k
+        t0 = time.perf_counter()
+        for _, op in jobs:
+            Z.ntt_std_device(src.data_ptr(), outs[k].data_ptr(), k, op, stream=sptr)
+            stream.synchronize()
+        t_plain = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        tks = [srv.submit(Z.make_job(src.data_ptr(), None, k, op)) for _, op in jobs]
+        for t in tks:
+            srv.wait(t)
+            srv.release(t)
+        t_served = time.perf_counter() - t0
+        per_k.append({"log_n": k, "plain_elem_per_s": el / t_plain, "served_elem_per_s": el / t_served,
+                      "ratio": t_plain / t_served, "share_of_mix_elements": (1 << k) / sum(1 << q for q in range(14, 23))})
+    res["per_k"] = per_k
     st = srv.stats()
     srv.close()
     res["server"] = {"iterations": st["iterations"], "peak_in_use_bytes": st["peak_in_use_bytes"],
