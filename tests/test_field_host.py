@@ -120,6 +120,35 @@ def test_pack_unpack_canonical(shim):
         assert sum(int(q) << (32 * i) for i, q in enumerate(out)) == v % R
 
 
+def test_canonical_extremes(shim):
+    """to_canonical_words over its whole contract (limbs 0..7 <= 0xC0000000,
+    the multiplier's operand bound, and V < 2^261): every multiple k r of r with k r < 2^261 and its neighbours,
+    the top of the range, and random unnormalized limb vectors."""
+    out = (ctypes.c_uint32 * 8)()
+
+    def canon(l):
+        shim.shim_to_canonical(arr32(l), out)
+        return sum(int(q) << (32 * i) for i, q in enumerate(out))
+
+    top = 1 << 261
+    for k in range(top // R + 1):
+        for v in (k * R - 1, k * R, k * R + 1, k * R + R // 2):
+            if 0 <= v < top:
+                assert canon(limbs(v)) == v % R, (k, v)
+    for v in (0, 1, top - 1, top - R, R - 1):
+        assert canon(limbs(v)) == v % R
+    rnd = random.Random(11)
+    checked = 0
+    while checked < 3000:
+        l = [rnd.choice((rnd.randrange(0xC0000001), 0xC0000000, M29, 0)) for _ in range(8)]
+        l.append(rnd.randrange(1 << 29))
+        v = val(l)
+        if v >= top:
+            continue
+        assert canon(l) == v % R, l
+        checked += 1
+
+
 def test_host_field_against_vectors(shim):
     vec = json.load(open(os.path.join(HERE, "golden", "fr_vectors.json")))
     u64 = lambda h: (ctypes.c_uint64 * 4)(*[int(x, 16) for x in h])

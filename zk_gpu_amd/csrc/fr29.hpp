@@ -24,7 +24,7 @@
 //                   out limbs l[i] <= a[i] + 2^30 (i < 8).
 //   normalize(a):   carry propagation; value unchanged.
 //   reduce_value(a): a normalized, V < 2^261 -> normalized, V < 3r.
-//   to_canonical_words(a): a normalized -> canonical u32[8] words.
+//   to_canonical_words(a): limbs 0..7 <= MUL_A_LIMB_MAX, V < 2^261 -> canonical u32[8] words.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -178,7 +178,7 @@ ZK_HD void reduce_value(fe& a) {
 
 // bit offset helpers for packing 9 x 29 <-> 8 x 32
 ZK_HD void pack_words(uint32_t w[8], const fe& a) {
-  // requires normalized a with value < 2^256
+  // normalized a: the words of V mod 2^256
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
     const int i = (32 * q) / 29, s = 32 * q - 29 * i;
@@ -200,25 +200,51 @@ ZK_HD fe unpack_words(const uint32_t w[8]) {
   return a;
 }
 
+// a - b - borrow_in on 32-bit words, borrow out in {0, 1}. On the device a
+// chain of these is one v_sub_co_u32 / v_subb_co_u32 sequence through VCC.
+ZK_HD uint32_t sub_borrow(uint32_t a, uint32_t b, uint32_t bin, uint32_t* bout) {
+#if defined(__clang__)
+  unsigned int o;
+  const uint32_t d = __builtin_subc(a, b, bin, &o);
+  *bout = o;
+  return d;
+#else
+  const uint64_t t = (uint64_t)a - b - bin;
+  *bout = (uint32_t)(t >> 63);
+  return (uint32_t)t;
+#endif
+}
+
 // words (value < 2^256): subtract r once if the value is >= r
 ZK_HD void cond_sub_r_words(uint32_t w[8]) {
   uint32_t t[8];
   uint32_t bw = 0;
 #pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    const uint64_t d = (uint64_t)w[i] - p32(i) - bw;
-    t[i] = (uint32_t)d;
-    bw = (uint32_t)(d >> 63);
-  }
+  for (int i = 0; i < 8; ++i) t[i] = sub_borrow(w[i], p32(i), bw, &bw);
 #pragma unroll
   for (int i = 0; i < 8; ++i) w[i] = bw ? w[i] : t[i];
 }
 
-// normalized a (V < 2^261) -> canonical words of V mod r
+// a with limbs 0..7 <= MUL_A_LIMB_MAX (the multiplier's operand bound, so
+// normalisation cannot overflow a limb) and V < 2^261 -> canonical words of
+// V mod r.
+// After normalisation the top limb is L = floor(V / 2^232) < 2^29, and
+// q = floor(L * QMAGIC / 2^32) satisfies q <= L / (r8 + 1) <= V / r and
+// L - q r8 < L / (r8 + 1) + L r8 / 2^32 + r8 < 1.13 r8 (r8 = r's top limb),
+// so 0 <= V - q r < 1.13 r + 2^232 < 2r. As 2r < 2^256 the subtraction is
+// exact on the low 256 bits: pack them, subtract q r mod 2^256 (one
+// multiply-add per word and a borrow chain), then subtract r once if needed.
 ZK_HD void to_canonical_words(uint32_t w[8], fe a) {
-  reduce_value(a);
-  pack_words(w, a);  // V < 3r < 2^256
-  cond_sub_r_words(w);
+  normalize(a);
+  const uint32_t q = (uint32_t)(((uint64_t)a.l[NLIMBS - 1] * QMAGIC) >> 32);
+  pack_words(w, a);  // V mod 2^256
+  uint32_t c = 0, bw = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const uint64_t t = (uint64_t)q * p32(i) + c;
+    c = (uint32_t)(t >> 32);
+    w[i] = sub_borrow(w[i], (uint32_t)t, bw, &bw);
+  }
   cond_sub_r_words(w);
 }
 
