@@ -25,7 +25,7 @@ extern "C" {
 
 #define ZKGPU_OK 0
 #define ZKGPU_ERR_INVALID 1      /* bad argument (size, pointer, op, device) */
-#define ZKGPU_ERR_UNSUPPORTED 2  /* entry point not provided (MSM) */
+#define ZKGPU_ERR_UNSUPPORTED 2  /* reserved: no entry point returns it (kept so codes stay stable) */
 #define ZKGPU_ERR_HIP 3          /* HIP runtime error */
 #define ZKGPU_ERR_NOMEM 4        /* device allocation failed */
 #define ZKGPU_ERR_NONCANONICAL 5 /* an input element is >= r (host-buffer entry points, validation on) */
