@@ -140,11 +140,7 @@ ZK_HD void cond_sub_q_words(uint32_t w[8]) {
   uint32_t t[8];
   uint32_t bw = 0;
 #pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    const uint64_t d = (uint64_t)w[i] - p32(i) - bw;
-    t[i] = (uint32_t)d;
-    bw = (uint32_t)(d >> 63);
-  }
+  for (int i = 0; i < 8; ++i) t[i] = sub_borrow(w[i], p32(i), bw, &bw);
 #pragma unroll
   for (int i = 0; i < 8; ++i) w[i] = bw ? w[i] : t[i];
 }
@@ -246,6 +242,116 @@ ZK_HD g1j g1_add_mixed(const g1j& p, const g1a& q) {
   o.y = q_sub(q_mul(R, q_sub(V, o.x)), q_dbl(q_mul(p.y, J)));
   o.z = q_sub(q_sub(q_sqr(q_add(p.z, H)), Z1Z1), HH);
   return o;
+}
+
+namespace fq {
+// C q in the borrowed limb form of zkgpu::SubConst (limbs 0..7 >= 2^29 - 1)
+template <uint32_t C>
+struct SubQ {
+  static constexpr uint64_t limb_full(int i) {
+    uint64_t carry = 0, v = 0;
+    for (int k = 0; k <= i; ++k) {
+      const uint64_t t = (uint64_t)P29[k] * C + carry;
+      v = (k == NLIMBS - 1) ? t : (t & LIMB_MASK);
+      carry = t >> LIMB_BITS;
+    }
+    return v;
+  }
+  static constexpr uint32_t k(int i) {
+    return i == 0 ? (uint32_t)(limb_full(0) + (1u << LIMB_BITS))
+         : i < NLIMBS - 1 ? (uint32_t)(limb_full(i) + (1u << LIMB_BITS) - 1u)
+         : (uint32_t)(limb_full(i) - 1u);
+  }
+};
+// a + C q - b, no carries: b normalized with V_b < (C - 1) q; limbs 0..7 of
+// the result <= a[i] + 2^30, value V_a + C - V_b (units of q).
+template <uint32_t C>
+ZK_HD fe q_sub_lazy(const fe& a, const fe& b) {
+  fe o;
+#pragma unroll
+  for (int i = 0; i < NLIMBS; ++i) o.l[i] = a.l[i] + (SubQ<C>::k(i) - b.l[i]);
+  return o;
+}
+ZK_HD fe q_norm(fe a) {
+  normalize(a);
+  return a;
+}
+// a normalized with V < 3q: a == 0 mod q, i.e. a is 0, q or 2q limb for limb
+ZK_HD bool q_is_zero3(const fe& a) {
+  uint32_t z0 = 0, z1 = 0, z2 = 0;
+  uint64_t c2 = 0;
+#pragma unroll
+  for (int i = 0; i < NLIMBS; ++i) {
+    const uint64_t t = 2ull * P29[i] + c2;  // limbs of 2q, carried at compile time
+    const uint32_t q2 = (i < NLIMBS - 1) ? (uint32_t)(t & LIMB_MASK) : (uint32_t)t;
+    c2 = t >> LIMB_BITS;
+    z0 |= a.l[i];
+    z1 |= a.l[i] ^ P29[i];
+    z2 |= a.l[i] ^ q2;
+  }
+  return z0 == 0 || z1 == 0 || z2 == 0;
+}
+}  // namespace fq
+
+// The MSM's bucket sum: acc += (neg ? -q : q), with the identity carried in
+// *inf (acc is meaningful only when !*inf). Invariant between calls: acc's
+// X, Y, Z normalized with value < 3q. madd-2007-bl with Z3 = 2 Z1 H (one
+// product instead of a square and two subtractions) and lazy field steps;
+// bounds in units of q, from mul's V_out < 1 + V_a V_b q / 2^261 with
+// q / 2^261 < 1/169 (operand a: limbs <= MUL_A_LIMB_MAX; b: normalized):
+//   y2 (q.y or 3q - q.y, limbs < 2^30)  < 3
+//   Z1Z1 < 1.06, Z1 Z1Z1 < 1.02, U2 < 1.01, S2 = y2 Z1 Z1Z1 < 1.02
+//   H = U2 + 4q - X1 (normalized)  < 5.02;  R = 2 (S2 + 4q - Y1) (normalized) < 10.04
+//   HH < 1.15, I = 4 HH (normalized) < 4.61, J = H I < 1.14, V = X1 I < 1.09, R^2 < 1.61
+//   X3 = R^2 + 3q - J + 3q - V + 3q - V (normalized) < 10.61
+//   V + 12q - X3 (limbs < 2^29 + 2^30) < 13.09, R (V - X3) < 1.78, Y1 J < 1.03
+//   Y3 = R (V - X3) + 3q - Y1 J + 3q - Y1 J < 7.8;  Z3 = 2 Z1 H < 2.19
+// X3 and Y3 are normalized and value-reduced (below 3q), Z3 normalized.
+// Z1 != 0, so Z3 == 0 exactly when H == 0: then P = +-Q, and the sum is 2P
+// (R == 0) or the identity.
+ZK_HD void g1_bucket_add(g1j& acc, bool& inf, const g1a& q, bool neg) {
+  using namespace fq;
+  if (q.inf) return;
+  const fe yn = q_sub_lazy<3>(q_zero(), q.y);
+  fe y2;
+#pragma unroll
+  for (int i = 0; i < NLIMBS; ++i) y2.l[i] = neg ? yn.l[i] : q.y.l[i];
+  if (inf) {
+    acc.x = q.x;
+    acc.y = q_norm(y2);
+    acc.z = q_from_const(fq::ONE261);
+    inf = false;
+    return;
+  }
+  const fe Z1Z1 = q_mul(acc.z, acc.z);
+  const fe U2 = q_mul(q.x, Z1Z1);
+  const fe S2 = q_mul(y2, q_mul(acc.z, Z1Z1));
+  const fe H = q_norm(q_sub_lazy<4>(U2, acc.x));
+  const fe d = q_sub_lazy<4>(S2, acc.y);
+  const fe R = q_norm(zkgpu::add(d, d));
+  const fe HH = q_mul(H, H);
+  fe I = HH;
+#pragma unroll
+  for (int i = 0; i < NLIMBS; ++i) I.l[i] <<= 2;
+  normalize(I);
+  const fe J = q_mul(H, I);
+  const fe V = q_mul(acc.x, I);
+  const fe X3 = q_norm(q_sub_lazy<3>(q_sub_lazy<3>(q_sub_lazy<3>(q_mul(R, R), J), V), V));
+  const fe YJ = q_mul(acc.y, J);
+  fe Y3 = q_norm(q_sub_lazy<3>(q_sub_lazy<3>(q_mul(q_sub_lazy<12>(V, X3), R), YJ), YJ));
+  const fe ZH = q_mul(acc.z, H);
+  const fe Z3 = q_norm(zkgpu::add(ZH, ZH));
+  if (q_is_zero3(Z3)) {
+    if (q_is_zero(R)) acc = g1_double(acc);
+    else inf = true;
+    return;
+  }
+  fe X = X3;
+  q_reduce(X);
+  q_reduce(Y3);
+  acc.x = X;
+  acc.y = Y3;
+  acc.z = Z3;
 }
 
 // P + Q, add-2007-bl: 11M + 5S

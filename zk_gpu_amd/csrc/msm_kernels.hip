@@ -220,12 +220,12 @@ __global__ void msm_buckets_kernel(const BaseLimbs* __restrict__ blimbs, const u
   if (b >= nbuckets) return;
   const uint32_t start = offsets[b], cnt = counts[b];
   g1j acc = g1_identity();
+  bool inf = true;
   for (uint32_t k = 0; k < cnt; ++k) {
     const uint32_t e = list[start + k];
-    g1a p = load_base(blimbs, e & 0x7fffffffu);
-    if (e >> 31) p = g1_neg(p);
-    acc = g1_add_mixed(acc, p);
+    g1_bucket_add(acc, inf, load_base(blimbs, e & 0x7fffffffu), (e >> 31) != 0);
   }
+  if (inf) acc = g1_identity();
   store_point(buckets, b, acc);
 }
 
