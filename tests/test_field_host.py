@@ -275,3 +275,28 @@ def test_dft_regs_worst_case_bounds(shim, radix, chain):
             want = sum(x * pow(w, i * k, R) for i, x in enumerate(xs)) % R
             assert val(o) % R == want, (radix, chain, k, xs[:2])
             assert all(v <= 0xC0000000 for v in o) and val(o) <= 160 * R, (radix, k)
+
+
+def test_mont_cols_header_is_generated():
+    """The device multiplier's column chains (mont_cols.hpp) match their
+    generator, and the generator's columns hold every partial product of the
+    schoolbook (a_i b_j) and of the reduction (m_i p_j, j >= 1) exactly once,
+    in the column i + j they belong to."""
+    import importlib.util
+    gen = os.path.join(HERE, "..", "zk_gpu_amd", "csrc", "gen_mont_cols.py")
+    spec = importlib.util.spec_from_file_location("gen_mont_cols", gen)
+    G = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(G)
+    with open(G.OUT) as f:
+        assert f.read() == G.render(), "mont_cols.hpp is stale: run gen_mont_cols.py"
+    ab, mp = [], []
+    for k in range(1, 17):
+        for kind, i, j in G.column_terms(k):
+            assert i + j == k and 0 <= i < 9 and 0 <= j < 9
+            (ab if kind == "ab" else mp).append((i, j))
+            if kind == "mp":
+                assert i < k and j >= 1  # m_i exists before column k; m_i p_0 follows the digit
+    # column 0 (a_0 b_0) and the nine m_k p_0 terms are written outside the chains
+    assert sorted(ab + [(0, 0)]) == [(i, j) for i in range(9) for j in range(9)]
+    assert sorted(mp) == [(i, j) for i in range(9) for j in range(1, 9)]
+    assert len(ab) + 1 + len(mp) + 9 == 162

@@ -159,7 +159,7 @@ ZK_HD void dif_stage(fe* u, const uint4* roots, uint32_t root_log) {
       const fe x = u[b + j], y = u[b + j + H];
       u[b + j] = add(x, y);
       if constexpr (j == 0) u[b + j + H] = sub<C>(x, y);
-      else u[b + j + H] = mont_mul<CHAIN>(sub<C>(x, y), w);
+      else u[b + j + H] = mont_mul_u<CHAIN>(sub<C>(x, y), w);
     });
   });
 }

@@ -62,25 +62,33 @@ ZK_HD uint32_t p32(int i) {
        : i == 5 ? P32[5] : i == 6 ? P32[6] : P32[7];
 }
 
-// Keeps a column accumulator opaque to the compiler's reassociation, so that
-// a column is one multiply-add chain that starts from the carry.
-template <bool CHAIN>
-ZK_HD void chain_point(uint64_t& acc) {
+// r for the column-asm multiplier
+struct FrMod {
+  static constexpr uint32_t ninv = NINV29;
+  static constexpr uint32_t p(int i) { return P29[i]; }
+};
+
+}  // namespace zkgpu
+
 #ifdef __HIP_DEVICE_COMPILE__
-  if constexpr (CHAIN) asm("" : "+v"(acc));
+#include "mont_cols.hpp"
 #endif
-  (void)acc;
-}
+
+namespace zkgpu {
 
 // Montgomery product a * b * 2^-261 mod r (not reduced), product scanning
 // with the reduction interleaved per column (FIPS, Koc-Acar-Kaliski 1996).
 // CHAIN = false: the compiler splits each column into chains that start from
 // zero and adds the carry last (more independent work per wave, one 64-bit
-// add more per column). CHAIN = true: one chain per column from the carry
-// (fewer instructions and registers, longer dependences; it pays where four
-// waves per SIMD hide them).
+// add more per column). CHAIN = true (device): each column is one chain of
+// multiply-adds from the carry, written as one asm statement per column
+// (mont_cols.hpp): fewer instructions and registers, longer dependences.
+// The host build computes the same product in plain C either way.
 template <bool CHAIN = false>
 ZK_HD fe mont_mul(const fe& a, const fe& b) {
+#ifdef __HIP_DEVICE_COMPILE__
+  if constexpr (CHAIN) return mont_mul_cols<FrMod>(a, b);
+#endif
   uint32_t m[NLIMBS];
   fe o;
   uint64_t acc = 0;
@@ -91,14 +99,12 @@ ZK_HD fe mont_mul(const fe& a, const fe& b) {
       const int j = k - i;
       if (j < 0 || j >= NLIMBS) continue;
       acc += (uint64_t)a.l[i] * b.l[j];
-      chain_point<CHAIN>(acc);
     }
 #pragma unroll
     for (int i = 0; i < NLIMBS; ++i) {
       const int j = k - i;
       if (i >= k || j < 1 || j >= NLIMBS) continue;
       acc += (uint64_t)m[i] * p29(j);
-      chain_point<CHAIN>(acc);
     }
     if (k < NLIMBS) {
       m[k] = ((uint32_t)acc * NINV29) & LIMB_MASK;
@@ -110,6 +116,16 @@ ZK_HD fe mont_mul(const fe& a, const fe& b) {
   }
   o.l[NLIMBS - 1] = (uint32_t)acc;
   return o;
+}
+
+// mont_mul with b the same in every lane of the wave (a root of a register
+// DFT, a uniform scale): the single-chain form takes b from SGPRs.
+template <bool CHAIN = false>
+ZK_HD fe mont_mul_u(const fe& a, const fe& b) {
+#ifdef __HIP_DEVICE_COMPILE__
+  if constexpr (CHAIN) return mont_mul_cols<FrMod, true>(a, b);
+#endif
+  return mont_mul<false>(a, b);
 }
 
 ZK_HD fe add(const fe& a, const fe& b) {

@@ -77,7 +77,8 @@ __device__ __forceinline__ void store_planar(void* base, uint32_t plane, uint32_
 //              instead of 72 KiB, so up to four share a CU), for two more
 //              barriers per exchange. Otherwise all nine limbs at once in the
 //              planes of store_planar (36 B per element).
-//   chain:     the multiplier's single-chain column form (mont_mul<true>).
+//   chain:     the multiplier's single-chain column form (mont_mul<true>,
+//              one asm chain of multiply-adds per column).
 //   le_max:    log2 of the elements each thread keeps (radix of the rounds).
 // The round twiddles are loaded after the barriers: loading them ahead,
 // across the barrier, spilled or ran slower in every configuration.
@@ -249,7 +250,7 @@ __device__ __forceinline__ void ntt_round(fe (&u)[E], const PassDesc& d, const T
               fe sc;
 #pragma unroll
               for (int q = 0; q < NLIMBS; ++q) sc.l[q] = d.scale[q];
-              v = mont_mul<X::chain>(v, sc);
+              v = mont_mul_u<X::chain>(v, sc);
             }
             uint32_t w[8];
             if (d.flags & PASS_CANON) {
@@ -333,11 +334,14 @@ constexpr int MAX_TILE_THREADS = 1 << (TILE_LOG_MAX - 3);
 //     second would need four waves per SIMD, where these tiles spill), and
 //     the tiles of passes too small to give a SIMD more than two waves (any
 //     transform below 2^21 elements): the one-phase exchange (two barriers
-//     per round fewer), the two-wave register budget (256 VGPRs; they need
-//     165-174, no spills) and split columns, whose independent chains hide
-//     latency that two waves cannot.
+//     per round fewer) and the two-wave register budget (256 VGPRs; they
+//     need 132-137, no spills).
+// Both take the single-chain multiplier (one asm chain of multiply-adds per
+// column): with two waves per SIMD it still beat the compiler's split
+// columns (k=20 step 0.307 vs 0.320 ms), since two waves of eight
+// independent products per thread already hide the chain's latency.
 constexpr uint32_t NARROW_TILE_LOG = 11;
-using PassXchg = Xchg<false, false>;
+using PassXchg = Xchg<false, true>;
 using NarrowXchg = Xchg<true, true>;
 
 template <int LOGN>
@@ -366,7 +370,7 @@ __global__ void __launch_bounds__(1 << (NARROW_TILE_LOG - 3), 4) ntt_pass_kernel
 // 16-byte DS operations), then one word holding the claimed item.
 //
 // The persistent kernel inlines every tile length. With the two-phase
-// exchange and single-chain columns it needs 149 VGPRs (no spills), so three
+// exchange and single-chain columns it needs 157 VGPRs (no spills), so three
 // workgroups share a CU (three waves per SIMD, 3 x 40 KiB of LDS); with the
 // one-phase exchange and split columns it needed 234 and ran two.
 using BatchXchg = Xchg<true, true>;
