@@ -676,19 +676,28 @@ int Engine::prepare(const JobSpec& job, JobTables& t, hipStream_t s, int plan_ki
       if (launch_pow_table(t.roots->ptr, w, 1ull << (k - lnt), 1u << lnt, true, s) != hipSuccess) return ZK_ERR_HIP;
     }
   }
-  // inter-pass twiddles W_p[t * L + i] = omega_m^(t * i)
+  // inter-pass twiddles W_p[t * L + i] = omega_m^(t * i); for the inverse,
+  // the last boundary's also carry the divisor
+  const bool fold = fold_divisor(job.op, p);
+  uint32_t divw[8];
+  host::to_words32(divw, job.divisor);
   uint32_t log_m = k;
   for (uint32_t i = 0; i + 1 < p.npass; ++i) {
     const uint32_t log_L = log_m - p.digit[i];
+    const bool scaled = fold && i + 2 == p.npass;
     bool is_new = false;
-    t.w[i] = fetch(make_key('W', k, log_m, log_L, job.omega), (size_t)32 << log_m, &is_new);
+    const TableKey key = scaled ? make_key('V', k, log_m, log_L, job.omega, &job.divisor)
+                                : make_key('W', k, log_m, log_L, job.omega);
+    t.w[i] = fetch(key, (size_t)32 << log_m, &is_new);
     if (!t.w[i]) return ZK_ERR_NOMEM;
     if (is_new) {
       const uint32_t h = (log_m + 1) / 2;
       std::shared_ptr<DevBuf> lo, hi;
       int st = build_pow_lohi(job.omega, 1ull << (k - log_m), log_m, h, lo, hi, s);
       if (st) return st;
-      if (launch_twiddle_table(t.w[i]->ptr, lo->ptr, hi->ptr, h, log_m, log_L, s) != hipSuccess) return ZK_ERR_HIP;
+      if (launch_twiddle_table(t.w[i]->ptr, lo->ptr, hi->ptr, h, log_m, log_L, scaled ? divw : nullptr, s) !=
+          hipSuccess)
+        return ZK_ERR_HIP;
       temps.push_back(lo);
       temps.push_back(hi);
     }
@@ -715,7 +724,7 @@ int Engine::prepare(const JobSpec& job, JobTables& t, hipStream_t s, int plan_ki
     }
     (post ? t.post : t.pre) = tab;
   }
-  if (job.op == OP_INV) {
+  if (job.op == OP_INV && !fold) {
     t.uscale = true;
     host::to_limbs261(t.scale261, job.divisor);
   }

@@ -88,8 +88,9 @@ struct JobSpec {
   host::Fr coset{};    // COSET_FWD: a[i] *= coset^i first; COSET_INV: out[i] *= coset^i last
 };
 
-// Resident-table key: kind ('R' roots, 'W' twiddles, 'P' coset pre-scale,
-// 'Q' coset post-scale), sizes, and the field elements the table derives from.
+// Resident-table key: kind ('R' roots, 'W' twiddles, 'V' twiddles scaled by
+// the inverse's divisor, 'P' coset pre-scale, 'Q' coset post-scale), sizes,
+// and the field elements the table derives from.
 struct TableKey {
   char kind = 0;
   uint32_t a = 0, b = 0, c = 0;
@@ -334,8 +335,13 @@ hipError_t launch_pow_table(void* out, const uint32_t base[8], uint64_t step, ui
                             hipStream_t s);
 hipError_t launch_pow_table_2level(void* out, const void* lo, const void* hi, uint32_t h, uint64_t count,
                                    const uint32_t* scale, hipStream_t s);
+// scale (optional, halo2 words): a factor folded into every twiddle
 hipError_t launch_twiddle_table(void* out, const void* lo, const void* hi, uint32_t h, uint32_t log_m,
-                                uint32_t log_L, hipStream_t s);
+                                uint32_t log_L, const uint32_t* scale, hipStream_t s);
+// The inverse's divisor is folded into the twiddles of the last pass boundary
+// (which the last pass applies to every element on load) when the plan has
+// more than one pass; a one-pass inverse scales its outputs instead.
+inline bool fold_divisor(int op, const Plan& p) { return op == OP_INV && p.npass > 1; }
 hipError_t launch_fill_random(void* out, uint64_t seed, uint64_t start, uint64_t count, hipStream_t s);
 hipError_t launch_check_canonical(const void* in, uint64_t count, uint32_t* flag, hipStream_t s);
 hipError_t launch_persistent(const PassDesc* d_descs, const uint32_t* d_tile_start, uint32_t ndesc, uint32_t total,

@@ -122,14 +122,18 @@ hipError_t build_tables(const JobSpec& job, const Plan& p, const Layout& L, uint
   uint32_t w[8];
   host::to_words32(w, job.omega);
   hipError_t e = launch_pow_table(base + L.roots, w, 1ull << (k - lnt), 1u << lnt, true, s);
+  uint32_t divw[8];
+  host::to_words32(divw, job.divisor);
+  const bool fold = fold_divisor(job.op, p);
   uint32_t log_m = k;
   for (uint32_t i = 0; e == hipSuccess && i + 1 < p.npass; ++i) {
     const uint32_t log_L = log_m - p.digit[i];
     const uint32_t h = (log_m + 1) / 2;
+    const bool scaled = fold && i + 2 == p.npass;
     e = pow_lohi(job.omega, 1ull << (k - log_m), log_m, h, base + L.temp, s);
     if (e == hipSuccess)
       e = launch_twiddle_table(base + L.w[i], base + L.temp, base + L.temp + 48 * ((size_t)1 << h), h, log_m, log_L,
-                               s);
+                               scaled ? divw : nullptr, s);
     log_m = log_L;
   }
   if (e == hipSuccess && (job.op == OP_COSET_FWD || job.op == OP_COSET_INV)) {
@@ -194,7 +198,7 @@ int zkgpu_ntt_device_noalloc(int device_id, const void* d_in, void* d_out, uint3
   const bool coset_op = op == OP_COSET_FWD || op == OP_COSET_INV;
   const bool reuse = S.tables_valid && S.log_n == log_n && S.op == op && same_plan(S.plan, p) &&
                      same_fr(S.omega, job.omega) && (!coset_op || same_fr(S.coset, job.coset)) &&
-                     (op != OP_COSET_INV || same_fr(S.divisor, job.divisor));
+                     ((op != OP_COSET_INV && op != OP_INV) || same_fr(S.divisor, job.divisor));
   if (!reuse) {
     S.tables_valid = false;
     if (build_tables(job, p, L, S.ptr, s) != hipSuccess) {
@@ -215,7 +219,7 @@ int zkgpu_ntt_device_noalloc(int device_id, const void* d_in, void* d_out, uint3
   for (uint32_t i = 0; i + 1 < p.npass; ++i) t.w[i] = S.ptr + L.w[i];
   if (op == OP_COSET_FWD) t.pre = S.ptr + L.coset;
   if (op == OP_COSET_INV) t.post = S.ptr + L.coset;
-  if (op == OP_INV) {
+  if (op == OP_INV && !fold_divisor(op, p)) {
     t.uscale = true;
     host::to_limbs261(t.scale261, job.divisor);
   }

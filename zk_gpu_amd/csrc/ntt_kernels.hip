@@ -480,14 +480,17 @@ __global__ void pow_table_2level_kernel(void* out, const void* lo, const void* h
 }
 
 // Inter-pass twiddles of a sub-problem of size m = 2^log_m split as N x L:
-// out[t * L + i] = w_m^((t * i) mod m), via w_m^e = lo[e & (2^h-1)] * hi[e >> h].
+// out[t * L + i] = w_m^((t * i) mod m) (* scale), via w_m^e = lo[e & (2^h-1)] *
+// hi[e >> h]. The scale folds a uniform factor (the inverse's divisor) into
+// the twiddles the last pass applies to every element on load.
 __global__ void twiddle_table_kernel(void* out, const void* lo, const void* hi, uint32_t h, uint32_t log_m,
-                                     uint32_t log_L) {
+                                     uint32_t log_L, FeArg scale, int use_scale) {
   const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (1ull << log_m)) return;
   const uint64_t t = idx >> log_L, i = idx & ((1ull << log_L) - 1);
   const uint64_t e = (t * i) & ((1ull << log_m) - 1);
-  const fe v = mont_mul(load_limbs48(lo, e & ((1ull << h) - 1)), load_limbs48(hi, e >> h));
+  fe v = mont_mul(load_limbs48(lo, e & ((1ull << h) - 1)), load_limbs48(hi, e >> h));
+  if (use_scale) v = mont_mul(canon_fe(v), to_domain261(scale.w));
   uint32_t w[8];
   to_canonical_words(w, v);
   store_words(out, idx, w);
@@ -654,9 +657,11 @@ hipError_t launch_pow_table_2level(void* out, const void* lo, const void* hi, ui
   return hipGetLastError();
 }
 hipError_t launch_twiddle_table(void* out, const void* lo, const void* hi, uint32_t h, uint32_t log_m,
-                                uint32_t log_L, hipStream_t s) {
+                                uint32_t log_L, const uint32_t* scale, hipStream_t s) {
   const uint64_t m = 1ull << log_m;
-  twiddle_table_kernel<<<(uint32_t)((m + 255) / 256), 256, 0, s>>>(out, lo, hi, h, log_m, log_L);
+  uint32_t zero[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  twiddle_table_kernel<<<(uint32_t)((m + 255) / 256), 256, 0, s>>>(out, lo, hi, h, log_m, log_L,
+                                                                  to_arg(scale ? scale : zero), scale ? 1 : 0);
   return hipGetLastError();
 }
 hipError_t launch_fill_random(void* out, uint64_t seed, uint64_t start, uint64_t count, hipStream_t s) {
