@@ -19,10 +19,11 @@
 namespace zkgpu {
 
 hipError_t launch_msm_digits(const void* scalars, const void* bases, uint32_t n, uint32_t c, uint32_t windows,
-                             void* blimbs, int32_t* digits, uint32_t* ranks, uint32_t* counts, hipStream_t s);
-hipError_t launch_msm_scan(const uint32_t* counts, uint32_t m, uint32_t* sums, uint32_t* offsets, hipStream_t s);
-hipError_t launch_msm_scatter(const int32_t* digits, const uint32_t* ranks, uint32_t n, uint32_t c, uint32_t windows,
-                              const uint32_t* offsets, uint32_t* list, hipStream_t s);
+                             uint32_t lo_bits, void* blimbs, uint16_t* keys, uint32_t* part_counts, hipStream_t s);
+hipError_t launch_msm_sort(const uint16_t* keys, uint32_t n, uint32_t c, uint32_t windows, uint32_t lo_bits,
+                           const uint32_t* part_counts, uint32_t* part_off, uint32_t* part_cursor, uint32_t* t_entry,
+                           uint8_t* t_lo, uint32_t* counts, uint32_t* offsets, uint32_t* list, hipStream_t s);
+uint32_t msm_lo_bits(uint32_t c);
 hipError_t launch_msm_buckets(const void* blimbs, const uint32_t* offsets, const uint32_t* counts,
                               const uint32_t* list, uint32_t nbuckets, void* buckets, hipStream_t s);
 hipError_t launch_msm_chunks(const void* buckets, uint32_t half, uint32_t run, uint32_t runs_per_window,
@@ -177,9 +178,9 @@ Jac j_add(const Jac& p, const Jac& q) {  // add-2007-bl
 
 // Window size and workspace of an n-point MSM.
 struct MsmPlan {
-  uint32_t n = 0, c = 0, windows = 0, half = 0, run = 0, runs = 0;
-  size_t off_blimbs = 0, off_digits = 0, off_list = 0, off_counts = 0, off_offsets = 0, off_ranks = 0,
-         off_sums = 0, off_buckets = 0, off_parts = 0, off_wout = 0, bytes = 0;
+  uint32_t n = 0, c = 0, windows = 0, half = 0, run = 0, runs = 0, lo_bits = 0, sort_parts = 0;
+  size_t off_blimbs = 0, off_keys = 0, off_tentry = 0, off_tlo = 0, off_list = 0, off_counts = 0, off_offsets = 0,
+         off_pcounts = 0, off_poff = 0, off_pcursor = 0, off_buckets = 0, off_parts = 0, off_wout = 0, bytes = 0;
 };
 
 static size_t up256(size_t b) { return (b + 255) & ~(size_t)255; }
@@ -202,13 +203,21 @@ MsmPlan msm_plan(uint32_t n) {
     off += up256(bytes);
     return o;
   };
+  // the sort's radix split of a bucket: lo_bits (<= 8) within a partition,
+  // the rest selects one of sort_parts partitions per window
+  p.lo_bits = msm_lo_bits(p.c);
+  p.sort_parts = 1u << (p.c - 1 - p.lo_bits);
+  const size_t pairs = (size_t)p.windows * n;
   p.off_blimbs = take((size_t)n * msm_base_limb_bytes());
-  p.off_digits = take((size_t)p.windows * n * 4);
-  p.off_list = take((size_t)p.windows * n * 4);
+  p.off_keys = take(pairs * 2);
+  p.off_tentry = take(pairs * 4);
+  p.off_tlo = take(pairs);
+  p.off_list = take(pairs * 4);
   p.off_counts = take(nb * 4);
   p.off_offsets = take(nb * 4);
-  p.off_ranks = take((size_t)p.windows * n * 4);
-  p.off_sums = take(((nb + 1023) / 1024) * 4);
+  p.off_pcounts = take((size_t)p.windows * p.sort_parts * 4);
+  p.off_poff = take((size_t)p.windows * p.sort_parts * 4);
+  p.off_pcursor = take((size_t)p.windows * p.sort_parts * 4);
   p.off_buckets = take(nb * msm_point_limb_bytes());
   p.off_parts = take((size_t)p.windows * p.runs * msm_point_limb_bytes());
   p.off_wout = take((size_t)p.windows * 96);
@@ -222,7 +231,7 @@ int msm_enqueue(Engine* eng, const MsmPlan& p, const void* d_scalars, const void
                 hipStream_t s) {
   char* b = (char*)ws;
   const uint32_t nb = p.windows * p.half;
-  uint32_t* counts = (uint32_t*)(b + p.off_counts);
+  uint32_t* counts = (uint32_t*)(b + p.off_counts);  // written for every bucket by the sort
   const bool timing = eng->pass_timing();
   hipError_t e = hipSuccess;
   // one launch (or launch group), bracketed by events while timing is on
@@ -239,17 +248,17 @@ int msm_enqueue(Engine* eng, const MsmPlan& p, const void* d_scalars, const void
       eng->finish_timed(ea, eb, name);
     }
   };
-  step("reset_counters_kernel", [&] { return launch_reset_counters(counts, nb, s); });
+  uint32_t* pcounts = (uint32_t*)(b + p.off_pcounts);
+  step("reset_counters_kernel", [&] { return launch_reset_counters(pcounts, p.windows * p.sort_parts, s); });
   step("msm_digits_kernel", [&] {
-    return launch_msm_digits(d_scalars, d_bases, p.n, p.c, p.windows, b + p.off_blimbs, (int32_t*)(b + p.off_digits),
-                             (uint32_t*)(b + p.off_ranks), counts, s);
+    return launch_msm_digits(d_scalars, d_bases, p.n, p.c, p.windows, p.lo_bits, b + p.off_blimbs,
+                             (uint16_t*)(b + p.off_keys), pcounts, s);
   });
-  step("msm_scan (3 kernels)", [&] {
-    return launch_msm_scan(counts, nb, (uint32_t*)(b + p.off_sums), (uint32_t*)(b + p.off_offsets), s);
-  });
-  step("msm_scatter_kernel", [&] {
-    return launch_msm_scatter((const int32_t*)(b + p.off_digits), (const uint32_t*)(b + p.off_ranks), p.n, p.c,
-                              p.windows, (const uint32_t*)(b + p.off_offsets), (uint32_t*)(b + p.off_list), s);
+  step("msm_sort (3 kernels)", [&] {
+    return launch_msm_sort((const uint16_t*)(b + p.off_keys), p.n, p.c, p.windows, p.lo_bits, pcounts,
+                           (uint32_t*)(b + p.off_poff), (uint32_t*)(b + p.off_pcursor), (uint32_t*)(b + p.off_tentry),
+                           (uint8_t*)(b + p.off_tlo), counts, (uint32_t*)(b + p.off_offsets),
+                           (uint32_t*)(b + p.off_list), s);
   });
   step("msm_buckets_kernel", [&] {
     return launch_msm_buckets(b + p.off_blimbs, (const uint32_t*)(b + p.off_offsets), counts,

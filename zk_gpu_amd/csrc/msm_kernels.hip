@@ -8,12 +8,10 @@
 // and 127 buckets in its own stack (cuda/fft.cu:158-202); here the buckets of
 // all points are shared:
 //
-//   msm_digits    one thread per point: base to limb form, scalar to signed
-//                 digits, bucket histogram (one atomic per digit, whose old
-//                 value is the point's rank in its bucket)
-//   scan          exclusive prefix sum of the bucket counts (three launches)
-//   msm_scatter   one thread per point: its index (sign in bit 31) into each
-//                 bucket's list, at the bucket's offset plus its rank
+//   msm_digits    per point: base to limb form, scalar to signed digits
+//                 (one 16-bit key per window)
+//   sort          the (window, point) pairs into bucket order: two radix
+//                 passes through LDS (partition scan, partition, bucket sort)
 //   msm_buckets   one thread per bucket: sum of its points (mixed additions)
 //   msm_chunks    one thread per run of L buckets of a window: running sums
 //                 give sum_j (j+1) B_j over the run
@@ -23,6 +21,8 @@
 // The windows are combined (Horner, c doublings each) by the host.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <algorithm>
 
 #include "fq29.hpp"
 
@@ -88,76 +88,92 @@ __device__ __forceinline__ fe fr_from_mont_factor() {
   return f;
 }
 
-// Per point: base in limb form; signed digits of the scalar; bucket counts.
-// digits[w * n + i] is the digit of window w (0 for identity bases),
-// ranks[w * n + i] the point's rank among that bucket's points.
-__global__ void msm_digits_kernel(const uint4* __restrict__ scalars, const uint4* __restrict__ bases, uint32_t n,
-                                  uint32_t c, uint32_t windows, BaseLimbs* __restrict__ blimbs,
-                                  int32_t* __restrict__ digits, uint32_t* __restrict__ ranks,
-                                  uint32_t* __restrict__ counts) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint4 bx0 = bases[4 * (size_t)i], bx1 = bases[4 * (size_t)i + 1];
-  const uint4 by0 = bases[4 * (size_t)i + 2], by1 = bases[4 * (size_t)i + 3];
-  const uint32_t xw[8] = {bx0.x, bx0.y, bx0.z, bx0.w, bx1.x, bx1.y, bx1.z, bx1.w};
-  const uint32_t yw[8] = {by0.x, by0.y, by0.z, by0.w, by1.x, by1.y, by1.z, by1.w};
-  uint32_t any = 0;
-#pragma unroll
-  for (int k = 0; k < 8; ++k) any |= xw[k] | yw[k];
-  const bool inf = any == 0;  // halo2's affine identity: x = y = 0
-  BaseLimbs e;
-  const fe x = fq::q_from_words(xw), y = fq::q_from_words(yw);
-  fe_to3(x, e.x0, e.x1, e.x2);
-  fe_to3(y, e.y0, e.y1, e.y2);
-  if (inf) e.y2.x |= 0x80000000u;
-  blimbs[i] = e;
-  // canonical scalar
-  const uint4 s0 = scalars[2 * (size_t)i], s1 = scalars[2 * (size_t)i + 1];
-  const uint32_t sw[8] = {s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, s1.z, s1.w};
-  uint32_t cw[8];
-  to_canonical_words(cw, mont_mul(unpack_words(sw), fr_from_mont_factor()));
+// Sorting the (window, point) pairs by bucket. A pair's key is the bucket
+// |d| - 1 of its digit d (c - 1 bits) with the sign in bit 15, MSM_KEY_NONE
+// for a zero digit (or an identity base). The bucket splits into hi = key >>
+// lo_bits (the partition, up to MSM_MAX_PARTS per window) and lo (the bucket
+// within its partition, lo_bits <= 8). Two radix passes, each staged through
+// LDS so that writes leave in runs:
+//   msm_digits_kernel       keys, and per-window partition sizes (LDS
+//                           histograms, one global atomic per counter and
+//                           workgroup)
+//   msm_part_scan_kernel    partition offsets over all windows
+//   msm_partition_kernel    per 4096-pair chunk: local sort by partition,
+//                           one atomic per partition reserves the chunk's run
+//   msm_bucket_sort_kernel  per partition: bucket counts and offsets, then
+//                           the entries i | sign << 31 in bucket order
+constexpr uint32_t MSM_KEY_NONE = 0xFFFFu;
+constexpr uint32_t MSM_MAX_PARTS = 128;           // 2^(15 - 8): c <= 16
+constexpr uint32_t MSM_MAX_COUNTERS = 2048;       // windows x partitions, any c in 4..16
+constexpr uint32_t SORT_THREADS = 256;
+constexpr uint32_t SORT_PER_THREAD = 16;
+constexpr uint32_t SORT_CHUNK = SORT_THREADS * SORT_PER_THREAD;
+
+// Per point: base in limb form; signed digits of the scalar as keys
+// (keys[w * n + i]); partition sizes part_counts[w * parts + hi] (zeroed by
+// the caller). A grid-stride loop, so each workgroup's LDS histogram covers
+// many points.
+__global__ void __launch_bounds__(256) msm_digits_kernel(const uint4* __restrict__ scalars,
+                                                         const uint4* __restrict__ bases, uint32_t n, uint32_t c,
+                                                         uint32_t windows, uint32_t lo_bits, BaseLimbs* __restrict__ blimbs,
+                                                         uint16_t* __restrict__ keys, uint32_t* __restrict__ part_counts) {
+  __shared__ uint32_t hist[MSM_MAX_COUNTERS];
+  const uint32_t parts = 1u << (c - 1 - lo_bits);
+  const uint32_t nctr = windows * parts;
+  for (uint32_t k = threadIdx.x; k < nctr; k += blockDim.x) hist[k] = 0;
+  __syncthreads();
   const uint32_t half = 1u << (c - 1);
-  uint32_t carry = 0;
-  for (uint32_t w = 0; w < windows; ++w) {
-    uint32_t v = scalar_bits(cw, w * c, c) + carry;
-    int32_t d;
-    if (v > half) {
-      d = (int32_t)v - (int32_t)(1u << c);
-      carry = 1;
-    } else {
-      d = (int32_t)v;
-      carry = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint4 bx0 = bases[4 * (size_t)i], bx1 = bases[4 * (size_t)i + 1];
+    const uint4 by0 = bases[4 * (size_t)i + 2], by1 = bases[4 * (size_t)i + 3];
+    const uint32_t xw[8] = {bx0.x, bx0.y, bx0.z, bx0.w, bx1.x, bx1.y, bx1.z, bx1.w};
+    const uint32_t yw[8] = {by0.x, by0.y, by0.z, by0.w, by1.x, by1.y, by1.z, by1.w};
+    uint32_t any = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) any |= xw[k] | yw[k];
+    const bool inf = any == 0;  // halo2's affine identity: x = y = 0
+    BaseLimbs e;
+    const fe x = fq::q_from_words(xw), y = fq::q_from_words(yw);
+    fe_to3(x, e.x0, e.x1, e.x2);
+    fe_to3(y, e.y0, e.y1, e.y2);
+    if (inf) e.y2.x |= 0x80000000u;
+    blimbs[i] = e;
+    // canonical scalar
+    const uint4 s0 = scalars[2 * (size_t)i], s1 = scalars[2 * (size_t)i + 1];
+    const uint32_t sw[8] = {s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, s1.z, s1.w};
+    uint32_t cw[8];
+    to_canonical_words(cw, mont_mul(unpack_words(sw), fr_from_mont_factor()));
+    uint32_t carry = 0;
+    for (uint32_t w = 0; w < windows; ++w) {
+      const uint32_t v = scalar_bits(cw, w * c, c) + carry;
+      // signed digit: above half it becomes v - 2^c with a carry into the next window
+      const bool neg = v > half;
+      const uint32_t mag = neg ? (1u << c) - v : v;
+      carry = neg ? 1u : 0u;
+      uint32_t key = MSM_KEY_NONE;
+      if (mag != 0 && !inf) {
+        key = (mag - 1u) | (neg ? 0x8000u : 0u);
+        atomicAdd(&hist[w * parts + ((mag - 1u) >> lo_bits)], 1u);
+      }
+      keys[(size_t)w * n + i] = (uint16_t)key;
     }
-    if (inf) d = 0;
-    digits[(size_t)w * n + i] = d;
-    if (d != 0) ranks[(size_t)w * n + i] = atomicAdd(&counts[w * half + (uint32_t)(d < 0 ? -d : d) - 1u], 1u);
   }
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k < nctr; k += blockDim.x)
+    if (hist[k]) atomicAdd(&part_counts[k], hist[k]);
 }
 
-// Exclusive scan of counts[0..m): per block of 1024 entries its sum, then the
-// block sums (one workgroup), then each block's entries.
-constexpr uint32_t SCAN_BLOCK = 1024;
-__global__ void scan_block_sums_kernel(const uint32_t* __restrict__ in, uint32_t m, uint32_t* __restrict__ sums) {
+// Exclusive scan of the m <= MSM_MAX_COUNTERS partition sizes (window-major)
+// into part_off, and the same into part_cursor (the partition kernel's
+// reservation cursors). One workgroup of 256.
+__global__ void __launch_bounds__(256) msm_part_scan_kernel(const uint32_t* __restrict__ part_counts, uint32_t m,
+                                                            uint32_t* __restrict__ part_off,
+                                                            uint32_t* __restrict__ part_cursor) {
   __shared__ uint32_t part[256];
-  const uint32_t base = blockIdx.x * SCAN_BLOCK;
+  const uint32_t per = (m + 255) / 256;
+  const uint32_t lo = threadIdx.x * per, hi = min(m, lo + per);
   uint32_t s = 0;
-  for (uint32_t k = threadIdx.x; k < SCAN_BLOCK; k += 256)
-    if (base + k < m) s += in[base + k];
-  part[threadIdx.x] = s;
-  __syncthreads();
-  for (uint32_t off = 128; off > 0; off >>= 1) {
-    if (threadIdx.x < off) part[threadIdx.x] += part[threadIdx.x + off];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) sums[blockIdx.x] = part[0];
-}
-// exclusive scan of sums[0..nb), nb <= 256 * 64, in one workgroup of 256
-__global__ void scan_sums_kernel(uint32_t* sums, uint32_t nb) {
-  __shared__ uint32_t part[256];
-  const uint32_t per = (nb + 255) / 256;
-  const uint32_t lo = threadIdx.x * per, hi = min(nb, lo + per);
-  uint32_t s = 0;
-  for (uint32_t k = lo; k < hi; ++k) s += sums[k];
+  for (uint32_t k = lo; k < hi; ++k) s += part_counts[k];
   part[threadIdx.x] = s;
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -171,44 +187,137 @@ __global__ void scan_sums_kernel(uint32_t* sums, uint32_t nb) {
   __syncthreads();
   uint32_t run = part[threadIdx.x];
   for (uint32_t k = lo; k < hi; ++k) {
-    const uint32_t v = sums[k];
-    sums[k] = run;
-    run += v;
-  }
-}
-__global__ void scan_apply_kernel(const uint32_t* __restrict__ in, uint32_t m, const uint32_t* __restrict__ sums,
-                                  uint32_t* __restrict__ out) {
-  __shared__ uint32_t vals[SCAN_BLOCK];
-  const uint32_t base = blockIdx.x * SCAN_BLOCK;
-  for (uint32_t k = threadIdx.x; k < SCAN_BLOCK; k += 256) vals[k] = base + k < m ? in[base + k] : 0;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t run = sums[blockIdx.x];
-    for (uint32_t k = 0; k < SCAN_BLOCK; ++k) {
-      const uint32_t v = vals[k];
-      vals[k] = run;
-      run += v;
-    }
-  }
-  __syncthreads();
-  for (uint32_t k = threadIdx.x; k < SCAN_BLOCK; k += 256) {
-    if (base + k < m) out[base + k] = vals[k];
+    part_off[k] = run;
+    part_cursor[k] = run;
+    run += part_counts[k];
   }
 }
 
-// Each point's index (bit 31: negative digit) into the list of every bucket
-// it falls in.
-__global__ void msm_scatter_kernel(const int32_t* __restrict__ digits, const uint32_t* __restrict__ ranks, uint32_t n,
-                                   uint32_t c, uint32_t windows, const uint32_t* __restrict__ offsets,
-                                   uint32_t* __restrict__ list) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+// One workgroup per (chunk of SORT_CHUNK points, window): the chunk's pairs
+// sorted by partition in LDS, then written to the partitions' runs reserved
+// with one atomic each: t_entry = i | sign << 31, t_lo = the bucket's low bits.
+__global__ void __launch_bounds__(SORT_THREADS) msm_partition_kernel(const uint16_t* __restrict__ keys, uint32_t n,
+                                                                     uint32_t c, uint32_t lo_bits,
+                                                                     uint32_t* __restrict__ part_cursor,
+                                                                     uint32_t* __restrict__ t_entry,
+                                                                     uint8_t* __restrict__ t_lo) {
+  __shared__ uint32_t cnt[MSM_MAX_PARTS], off[MSM_MAX_PARTS], base[MSM_MAX_PARTS];
+  __shared__ uint32_t st_entry[SORT_CHUNK];
+  __shared__ uint8_t st_lo[SORT_CHUNK], st_part[SORT_CHUNK];
+  const uint32_t w = blockIdx.y, i0 = blockIdx.x * SORT_CHUNK;
+  const uint32_t parts = 1u << (c - 1 - lo_bits), lo_mask = (1u << lo_bits) - 1u;
+  for (uint32_t p = threadIdx.x; p < parts; p += SORT_THREADS) cnt[p] = 0;
+  __syncthreads();
+  uint32_t key[SORT_PER_THREAD], rank[SORT_PER_THREAD];
+#pragma unroll
+  for (uint32_t q = 0; q < SORT_PER_THREAD; ++q) {
+    const uint32_t i = i0 + q * SORT_THREADS + threadIdx.x;
+    key[q] = i < n ? keys[(size_t)w * n + i] : MSM_KEY_NONE;
+    rank[q] = key[q] != MSM_KEY_NONE ? atomicAdd(&cnt[(key[q] & 0x7fffu) >> lo_bits], 1u) : 0u;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (uint32_t p = 0; p < parts; ++p) {
+      off[p] = run;
+      run += cnt[p];
+    }
+  }
+  for (uint32_t p = threadIdx.x; p < parts; p += SORT_THREADS)
+    base[p] = cnt[p] ? atomicAdd(&part_cursor[w * parts + p], cnt[p]) : 0u;
+  __syncthreads();
+#pragma unroll
+  for (uint32_t q = 0; q < SORT_PER_THREAD; ++q) {
+    if (key[q] == MSM_KEY_NONE) continue;
+    const uint32_t b = key[q] & 0x7fffu, p = b >> lo_bits;
+    const uint32_t j = off[p] + rank[q];
+    st_entry[j] = (i0 + q * SORT_THREADS + threadIdx.x) | ((key[q] & 0x8000u) << 16);
+    st_lo[j] = (uint8_t)(b & lo_mask);
+    st_part[j] = (uint8_t)p;
+  }
+  __syncthreads();
+  const uint32_t total = off[parts - 1] + cnt[parts - 1];
+  for (uint32_t j = threadIdx.x; j < total; j += SORT_THREADS) {
+    const uint32_t p = st_part[j];
+    const uint32_t pos = base[p] + (j - off[p]);
+    t_entry[pos] = st_entry[j];
+    t_lo[pos] = st_lo[j];
+  }
+}
+
+// One workgroup per partition (window w, partition p): the counts and list
+// offsets of its 2^lo_bits buckets (buckets w * half + (p << lo_bits) + lo),
+// then its entries into bucket order, a SORT_CHUNK chunk at a time sorted in
+// LDS so that each bucket's run is written by consecutive threads.
+__global__ void __launch_bounds__(SORT_THREADS) msm_bucket_sort_kernel(
+    const uint32_t* __restrict__ part_off, const uint32_t* __restrict__ part_counts,
+    const uint32_t* __restrict__ t_entry, const uint8_t* __restrict__ t_lo, uint32_t c, uint32_t lo_bits,
+    uint32_t* __restrict__ counts, uint32_t* __restrict__ offsets, uint32_t* __restrict__ list) {
+  __shared__ uint32_t cur[256], lcnt[256], loff[256];
+  __shared__ uint32_t st_entry[SORT_CHUNK];
+  __shared__ uint8_t st_lo[SORT_CHUNK];
+  const uint32_t g = blockIdx.x, hi_bits = c - 1 - lo_bits;
+  const uint32_t w = g >> hi_bits, p = g & ((1u << hi_bits) - 1u);
+  const uint32_t nlo = 1u << lo_bits;
+  const uint32_t start = part_off[g], m = part_counts[g];
+  for (uint32_t l = threadIdx.x; l < nlo; l += SORT_THREADS) lcnt[l] = 0;
+  __syncthreads();
+  for (uint32_t j = threadIdx.x; j < m; j += SORT_THREADS) atomicAdd(&lcnt[t_lo[start + j]], 1u);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = start;
+    for (uint32_t l = 0; l < nlo; ++l) {
+      cur[l] = run;
+      run += lcnt[l];
+    }
+  }
+  __syncthreads();
   const uint32_t half = 1u << (c - 1);
-  for (uint32_t w = 0; w < windows; ++w) {
-    const int32_t d = digits[(size_t)w * n + i];
-    if (d == 0) continue;
-    const uint32_t key = w * half + (uint32_t)(d < 0 ? -d : d) - 1u;
-    list[offsets[key] + ranks[(size_t)w * n + i]] = i | (d < 0 ? 0x80000000u : 0u);
+  for (uint32_t l = threadIdx.x; l < nlo; l += SORT_THREADS) {
+    const uint32_t b = w * half + (p << lo_bits) + l;
+    counts[b] = lcnt[l];
+    offsets[b] = cur[l];
+  }
+  for (uint32_t c0 = 0; c0 < m; c0 += SORT_CHUNK) {
+    const uint32_t len = min(SORT_CHUNK, m - c0);
+    __syncthreads();  // the previous chunk's reads of lcnt / loff / staging are done
+    for (uint32_t l = threadIdx.x; l < nlo; l += SORT_THREADS) lcnt[l] = 0;
+    __syncthreads();
+    uint32_t lo[SORT_PER_THREAD], rank[SORT_PER_THREAD], ent[SORT_PER_THREAD];
+#pragma unroll
+    for (uint32_t q = 0; q < SORT_PER_THREAD; ++q) {
+      const uint32_t j = q * SORT_THREADS + threadIdx.x;
+      if (j < len) {
+        lo[q] = t_lo[start + c0 + j];
+        ent[q] = t_entry[start + c0 + j];
+        rank[q] = atomicAdd(&lcnt[lo[q]], 1u);
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t run = 0;
+      for (uint32_t l = 0; l < nlo; ++l) {
+        loff[l] = run;
+        run += lcnt[l];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t q = 0; q < SORT_PER_THREAD; ++q) {
+      const uint32_t j = q * SORT_THREADS + threadIdx.x;
+      if (j < len) {
+        st_entry[loff[lo[q]] + rank[q]] = ent[q];
+        st_lo[loff[lo[q]] + rank[q]] = (uint8_t)lo[q];
+      }
+    }
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < len; j += SORT_THREADS) {
+      const uint32_t l = st_lo[j];
+      list[cur[l] + (j - loff[l])] = st_entry[j];
+    }
+    __syncthreads();
+    // advance each bucket's cursor past this chunk's entries
+    for (uint32_t l = threadIdx.x; l < nlo; l += SORT_THREADS) cur[l] += lcnt[l];
   }
 }
 
@@ -221,9 +330,20 @@ __global__ void msm_buckets_kernel(const BaseLimbs* __restrict__ blimbs, const u
   const uint32_t start = offsets[b], cnt = counts[b];
   g1j acc = g1_identity();
   bool inf = true;
+  // software pipeline: the list entry two ahead and the base one ahead are
+  // in flight while the current addition runs
+  uint32_t e = cnt > 0 ? list[start] : 0u;
+  uint32_t e_next = cnt > 1 ? list[start + 1] : 0u;
+  g1a base = load_base(blimbs, cnt > 0 ? e & 0x7fffffffu : 0u);
   for (uint32_t k = 0; k < cnt; ++k) {
-    const uint32_t e = list[start + k];
-    g1_bucket_add(acc, inf, load_base(blimbs, e & 0x7fffffffu), (e >> 31) != 0);
+    const g1a cur = base;
+    const bool neg = (e >> 31) != 0;
+    if (k + 1 < cnt) {
+      base = load_base(blimbs, e_next & 0x7fffffffu);
+      e = e_next;
+      e_next = k + 2 < cnt ? list[start + k + 2] : 0u;
+    }
+    g1_bucket_add(acc, inf, cur, neg);
   }
   if (inf) acc = g1_identity();
   store_point(buckets, b, acc);
@@ -336,23 +456,28 @@ hipError_t launch_fill_random_points(void* out, uint64_t count, uint64_t seed, u
 }
 
 hipError_t launch_msm_digits(const void* scalars, const void* bases, uint32_t n, uint32_t c, uint32_t windows,
-                             void* blimbs, int32_t* digits, uint32_t* ranks, uint32_t* counts, hipStream_t s) {
-  msm_digits_kernel<<<(n + 255) / 256, 256, 0, s>>>((const uint4*)scalars, (const uint4*)bases, n, c, windows,
-                                                    (BaseLimbs*)blimbs, digits, ranks, counts);
+                             uint32_t lo_bits, void* blimbs, uint16_t* keys, uint32_t* part_counts, hipStream_t s) {
+  if (c < 2 || c > 16 || lo_bits > 8 || lo_bits > c - 1 || windows * (1u << (c - 1 - lo_bits)) > MSM_MAX_COUNTERS)
+    return hipErrorInvalidValue;  // the kernels' LDS arrays are sized for these bounds
+  const uint32_t grid = std::min<uint32_t>((n + 255) / 256, 2048u);
+  msm_digits_kernel<<<grid, 256, 0, s>>>((const uint4*)scalars, (const uint4*)bases, n, c, windows, lo_bits,
+                                         (BaseLimbs*)blimbs, keys, part_counts);
   return hipGetLastError();
 }
-hipError_t launch_msm_scan(const uint32_t* counts, uint32_t m, uint32_t* sums, uint32_t* offsets, hipStream_t s) {
-  const uint32_t nb = (m + SCAN_BLOCK - 1) / SCAN_BLOCK;
-  scan_block_sums_kernel<<<nb, 256, 0, s>>>(counts, m, sums);
-  scan_sums_kernel<<<1, 256, 0, s>>>(sums, nb);
-  scan_apply_kernel<<<nb, 256, 0, s>>>(counts, m, sums, offsets);
+hipError_t launch_msm_sort(const uint16_t* keys, uint32_t n, uint32_t c, uint32_t windows, uint32_t lo_bits,
+                           const uint32_t* part_counts, uint32_t* part_off, uint32_t* part_cursor, uint32_t* t_entry,
+                           uint8_t* t_lo, uint32_t* counts, uint32_t* offsets, uint32_t* list, hipStream_t s) {
+  if (c < 2 || c > 16 || lo_bits > 8 || lo_bits > c - 1) return hipErrorInvalidValue;
+  const uint32_t parts = 1u << (c - 1 - lo_bits);
+  if (parts > MSM_MAX_PARTS || windows * parts > MSM_MAX_COUNTERS) return hipErrorInvalidValue;
+  msm_part_scan_kernel<<<1, 256, 0, s>>>(part_counts, windows * parts, part_off, part_cursor);
+  msm_partition_kernel<<<dim3((n + SORT_CHUNK - 1) / SORT_CHUNK, windows), SORT_THREADS, 0, s>>>(
+      keys, n, c, lo_bits, part_cursor, t_entry, t_lo);
+  msm_bucket_sort_kernel<<<windows * parts, SORT_THREADS, 0, s>>>(part_off, part_counts, t_entry, t_lo, c, lo_bits,
+                                                                  counts, offsets, list);
   return hipGetLastError();
 }
-hipError_t launch_msm_scatter(const int32_t* digits, const uint32_t* ranks, uint32_t n, uint32_t c, uint32_t windows,
-                              const uint32_t* offsets, uint32_t* list, hipStream_t s) {
-  msm_scatter_kernel<<<(n + 255) / 256, 256, 0, s>>>(digits, ranks, n, c, windows, offsets, list);
-  return hipGetLastError();
-}
+uint32_t msm_lo_bits(uint32_t c) { return std::min(8u, c - 1); }
 hipError_t launch_msm_buckets(const void* blimbs, const uint32_t* offsets, const uint32_t* counts,
                               const uint32_t* list, uint32_t nbuckets, void* buckets, hipStream_t s) {
   msm_buckets_kernel<<<(nbuckets + 63) / 64, 64, 0, s>>>((const BaseLimbs*)blimbs, offsets, counts, list, nbuckets,
