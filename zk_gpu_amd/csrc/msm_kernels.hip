@@ -8,7 +8,7 @@
 // and 127 buckets in its own stack (cuda/fft.cu:158-202); here the buckets of
 // all points are shared:
 //
-//   msm_digits    per point: base to limb form, scalar to signed digits
+//   msm_digits    per point: base to the 2^261 domain, scalar to signed digits
 //                 (one 16-bit key per window)
 //   sort          the (window, point) pairs into bucket order: two radix
 //                 passes through LDS (partition scan, partition, bucket sort)
@@ -28,10 +28,12 @@
 
 namespace zkgpu {
 
-// limb form of a base: x and y (2^261 domain) in 48-byte entries; bit 31 of
-// the y entry's last word marks the identity
+// a base as the bucket sums read it: x and y in the 2^261 domain as
+// canonical packed words (below q < 2^254), 64 bytes, so that each random
+// read is one aligned 64-byte segment; bit 31 of y's top word marks the
+// identity
 struct BaseLimbs {
-  uint4 x0, x1, x2, y0, y1, y2;
+  uint4 x0, x1, y0, y1;
 };
 // a bucket / partial sum: Jacobian X, Y, Z in 48-byte entries
 struct PointLimbs {
@@ -50,13 +52,15 @@ __device__ __forceinline__ void fe_to3(const fe& v, uint4& a, uint4& b, uint4& c
   b = make_uint4(v.l[4], v.l[5], v.l[6], v.l[7]);
   c = make_uint4(v.l[8], 0u, 0u, 0u);
 }
+// the base's coordinates as normalized limbs, value below q
 __device__ __forceinline__ g1a load_base(const BaseLimbs* b, uint32_t i) {
   const BaseLimbs e = b[i];
+  const uint32_t xw[8] = {e.x0.x, e.x0.y, e.x0.z, e.x0.w, e.x1.x, e.x1.y, e.x1.z, e.x1.w};
+  const uint32_t yw[8] = {e.y0.x, e.y0.y, e.y0.z, e.y0.w, e.y1.x, e.y1.y, e.y1.z, e.y1.w & 0x7fffffffu};
   g1a p;
-  p.x = fe_from3(e.x0, e.x1, e.x2);
-  p.y = fe_from3(e.y0, e.y1, e.y2);
-  p.inf = (e.y2.x >> 31) != 0;
-  p.y.l[8] &= 0x7fffffffu;
+  p.x = unpack_words(xw);
+  p.y = unpack_words(yw);
+  p.inf = (e.y1.w >> 31) != 0;
   return p;
 }
 __device__ __forceinline__ g1j load_point(const PointLimbs* p, uint32_t i) {
@@ -132,11 +136,14 @@ __global__ void __launch_bounds__(256) msm_digits_kernel(const uint4* __restrict
 #pragma unroll
     for (int k = 0; k < 8; ++k) any |= xw[k] | yw[k];
     const bool inf = any == 0;  // halo2's affine identity: x = y = 0
+    uint32_t xc[8], yc[8];
+    fq::q_canonical_words(xc, fq::q_from_words(xw));
+    fq::q_canonical_words(yc, fq::q_from_words(yw));
     BaseLimbs e;
-    const fe x = fq::q_from_words(xw), y = fq::q_from_words(yw);
-    fe_to3(x, e.x0, e.x1, e.x2);
-    fe_to3(y, e.y0, e.y1, e.y2);
-    if (inf) e.y2.x |= 0x80000000u;
+    e.x0 = make_uint4(xc[0], xc[1], xc[2], xc[3]);
+    e.x1 = make_uint4(xc[4], xc[5], xc[6], xc[7]);
+    e.y0 = make_uint4(yc[0], yc[1], yc[2], yc[3]);
+    e.y1 = make_uint4(yc[4], yc[5], yc[6], yc[7] | (inf ? 0x80000000u : 0u));
     blimbs[i] = e;
     // canonical scalar
     const uint4 s0 = scalars[2 * (size_t)i], s1 = scalars[2 * (size_t)i + 1];
