@@ -367,20 +367,25 @@ __global__ void __launch_bounds__(1 << (NARROW_TILE_LOG - 3), 4) ntt_pass_kernel
 //
 // LDS layout: the tiles' limb planes (two-phase exchange) from byte 0
 // (16-byte aligned, as in the single-pass kernel: the planes are accessed with
-// 16-byte DS operations), then one word holding the claimed item.
+// 16-byte DS operations). The claimed item is broadcast through word 0 of
+// the planes: when thread 0 writes it, every thread has passed the barrier
+// that follows the previous tile's last LDS read, and every thread reads it
+// before the barrier that precedes the next tile's first LDS write. So the
+// workgroup needs exactly its tile planes, 40 KiB, and four fill a CU.
 //
 // The persistent kernel inlines every tile length. With the two-phase
-// exchange and single-chain columns it needs 157 VGPRs (no spills), so three
-// workgroups share a CU (three waves per SIMD, 3 x 40 KiB of LDS); with the
-// one-phase exchange and split columns it needed 234 and ran two.
+// exchange and single-chain columns it fits the four-wave budget of 128
+// VGPRs (108 bytes spilled), so four workgroups share a CU (four waves per
+// SIMD); at three it needed 157 VGPRs; with the one-phase exchange and split
+// columns it needed 234 and ran two.
 using BatchXchg = Xchg<true, true>;
 constexpr uint32_t BATCH_TILE_LOG = 11;  // 2048-element tiles: 40 KiB of LDS
 constexpr int BATCH_THREADS = 1 << (BATCH_TILE_LOG - 3);
-constexpr uint32_t BATCH_ITEM_WORD = BatchXchg::words << BATCH_TILE_LOG;  // LDS word of the item slot
+constexpr uint32_t BATCH_ITEM_WORD = 0;  // LDS word of the item slot (inside the tile planes)
 
 constexpr uint32_t BATCH_THREADS_LOG = BATCH_TILE_LOG - 3;
 // resident workgroups per CU (one wave per SIMD each)
-constexpr int PERSIST_WGS = 3;
+constexpr int PERSIST_WGS = 4;
 
 // Item `group` of descriptor d: tiles group * G .. group * G + G - 1, G =
 // BATCH_THREADS / (threads per tile).
@@ -405,8 +410,8 @@ __global__ void __launch_bounds__(BATCH_THREADS, PERSIST_WGS) ntt_persistent_ker
                                                                        uint32_t* counter) {
   extern __shared__ __align__(16) uint32_t lds[];
   for (;;) {
-    // the slot lies past every tile's planes, and all reads of its previous
-    // value precede the second barrier of the previous iteration
+    // every LDS read of the previous tile, and of the slot's previous value,
+    // precedes a barrier that thread 0 has passed
     if (threadIdx.x == 0) lds[BATCH_ITEM_WORD] = atomicAdd(counter, 1u);
     __syncthreads();
     const uint32_t item = lds[BATCH_ITEM_WORD];
@@ -565,8 +570,8 @@ uint32_t batch_items(const PassDesc& d) {
   return (d.num_tiles + per_item - 1) / per_item;
 }
 uint32_t persistent_wgs_per_cu() { return (uint32_t)PERSIST_WGS; }
-// planes of the largest batch tile, then the 16-byte item slot
-static uint32_t persistent_lds_bytes() { return 4u * BATCH_ITEM_WORD + 16u; }
+// planes of the largest batch tile (the item slot lies inside them)
+static uint32_t persistent_lds_bytes() { return (4u * BatchXchg::words) << BATCH_TILE_LOG; }
 
 // Which pass-kernel family runs a descriptor (see ntt_pass_kernel_narrow).
 template <int LOGN>
