@@ -80,13 +80,15 @@ __device__ __forceinline__ void store_planar(void* base, uint32_t plane, uint32_
 //   chain:     the multiplier's single-chain column form (mont_mul<true>,
 //              one asm chain of multiply-adds per column).
 //   le_max:    log2 of the elements each thread keeps (radix of the rounds).
+//   swizzle:   LDS element index with bits 1-3 flipped by bits 4-6 (lds_elem)
 // The round twiddles are loaded after the barriers: loading them ahead,
 // across the barrier, spilled or ran slower in every configuration.
-template <bool TWO_PHASE, bool CHAIN, int LE_MAX = 3>
+template <bool TWO_PHASE, bool CHAIN, int LE_MAX = 3, bool SWIZZLE = false>
 struct Xchg {
   static constexpr bool two_phase = TWO_PHASE;
   static constexpr bool chain = CHAIN;
   static constexpr int le_max = LE_MAX;
+  static constexpr bool swizzle = SWIZZLE;
   static constexpr uint32_t words = TWO_PHASE ? 5u : (uint32_t)NLIMBS;  // LDS words per tile element
 };
 
@@ -128,11 +130,24 @@ struct TileCtx {
   bool active;
 };
 
+// LDS slot of tile element (column t.c, position pos). With SWZ, bits 1-3 of
+// the index are flipped by bits 4-6 (a bijection on every aligned group of
+// 128 elements, the identity on tiles under 16): the first round of a tile
+// with few columns writes elements 16 apart from consecutive threads, which
+// land in the same LDS banks unswizzled (a 16-byte slot per element and
+// plane, 256 bytes per bank row).
+template <bool SWZ>
+__device__ __forceinline__ uint32_t lds_elem(const TileCtx& t, uint32_t pos) {
+  const uint32_t i = t.c + (pos << t.logC);
+  if constexpr (SWZ) return i ^ ((i >> 3) & 0xEu);
+  return i;
+}
+
 // LDS exchange helpers (PH as in lds_put). A round of radix R = 2^LR at
 // stride 2^logp writes its register s*R + bitrev(m) to tile position
 // base(s) + m*2^logp; the next round reads its register s*R + j from
 // position iv(s) + j*N/R.
-template <int E, int R, int PH>
+template <int E, int R, int PH, bool SWZ>
 __device__ __forceinline__ void lds_write_round(const fe (&u)[E], const TileCtx& t, uint32_t logp, uint32_t* lds) {
   constexpr int S = E / R;
   constexpr int LR = Log2<R>::value;
@@ -147,11 +162,11 @@ __device__ __forceinline__ void lds_write_round(const fe (&u)[E], const TileCtx&
     static_for<R>([&](auto mc) {
       constexpr int m = decltype(mc)::value;
       const uint32_t pos = base + ((uint32_t)m << logp);
-      lds_put<PH>(lds, t.plane, t.c + (pos << t.logC), u[s * R + bitrev_c(m, LR)]);
+      lds_put<PH>(lds, t.plane, lds_elem<SWZ>(t, pos), u[s * R + bitrev_c(m, LR)]);
     });
   });
 }
-template <int E, int R, int PH>
+template <int E, int R, int PH, bool SWZ>
 __device__ __forceinline__ void lds_read_round(fe (&u)[E], const TileCtx& t, const uint32_t* lds) {
   constexpr int S = E / R;
   constexpr int LR = Log2<R>::value;
@@ -163,7 +178,7 @@ __device__ __forceinline__ void lds_read_round(fe (&u)[E], const TileCtx& t, con
     const uint32_t iv = t.i + s * NE;
     static_for<R>([&](auto jc) {
       constexpr int j = decltype(jc)::value;
-      lds_get<PH>(lds, t.plane, t.c + ((iv + j * NR) << t.logC), u[s * R + j]);
+      lds_get<PH>(lds, t.plane, lds_elem<SWZ>(t, iv + j * NR), u[s * R + j]);
     });
   });
 }
@@ -205,15 +220,15 @@ __device__ __forceinline__ void ntt_round(fe (&u)[E], const PassDesc& d, const T
     auto twiddle = [&](int s, int j) {
       return load_limbs48(roots, (uint64_t)(j * ((t.i + s * NE) & pmask)) << sh);
     };
-    if (t.active) lds_read_round<E, R, X::two_phase ? 0 : 2>(u, t, lds);
+    if (t.active) lds_read_round<E, R, X::two_phase ? 0 : 2, X::swizzle>(u, t, lds);
     __syncthreads();  // every LDS read precedes the next overwrite
     if constexpr (X::two_phase) {
       // limbs 5-8: the previous round (radix E, since only a tile's last
       // round is shorter) writes them where it wrote limbs 0-4, this round
       // reads them into the registers that now hold its limbs 0-4
-      if (t.active) lds_write_round<E, E, 1>(u, t, logp - LE, lds);
+      if (t.active) lds_write_round<E, E, 1, X::swizzle>(u, t, logp - LE, lds);
       __syncthreads();
-      if (t.active) lds_read_round<E, R, 1>(u, t, lds);
+      if (t.active) lds_read_round<E, R, 1, X::swizzle>(u, t, lds);
       __syncthreads();  // every LDS read of this round precedes any overwrite
     }
     if (t.active) {
@@ -262,7 +277,7 @@ __device__ __forceinline__ void ntt_round(fe (&u)[E], const PassDesc& d, const T
             store_words(d.out, off, w);
           }
         } else {
-          lds_put<X::two_phase ? 0 : 2>(lds, t.plane, t.c + (pos << t.logC), v);
+          lds_put<X::two_phase ? 0 : 2>(lds, t.plane, lds_elem<X::swizzle>(t, pos), v);
         }
       });
     });
@@ -339,9 +354,13 @@ constexpr int MAX_TILE_THREADS = 1 << (TILE_LOG_MAX - 3);
 // Both take the single-chain multiplier (one asm chain of multiply-adds per
 // column): with two waves per SIMD it still beat the compiler's split
 // columns (k=20 step 0.307 vs 0.320 ms), since two waves of eight
-// independent products per thread already hide the chain's latency.
+// independent products per thread already hide the chain's latency. The
+// wide kernel swizzles its LDS slots: its tiles with two columns (k=20 pass
+// 2) lost 16 times more cycles to bank conflicts without it; the narrow
+// kernel's eight-column tiles conflict little and the swizzle only cost
+// instructions there.
 constexpr uint32_t NARROW_TILE_LOG = 11;
-using PassXchg = Xchg<false, true>;
+using PassXchg = Xchg<false, true, 3, true>;
 using NarrowXchg = Xchg<true, true>;
 
 template <int LOGN>
@@ -378,7 +397,7 @@ __global__ void __launch_bounds__(1 << (NARROW_TILE_LOG - 3), 4) ntt_pass_kernel
 // VGPRs (108 bytes spilled), so four workgroups share a CU (four waves per
 // SIMD); at three it needed 157 VGPRs; with the one-phase exchange and split
 // columns it needed 234 and ran two.
-using BatchXchg = Xchg<true, true>;
+using BatchXchg = Xchg<true, true, 3, true>;
 constexpr uint32_t BATCH_TILE_LOG = 11;  // 2048-element tiles: 40 KiB of LDS
 constexpr int BATCH_THREADS = 1 << (BATCH_TILE_LOG - 3);
 constexpr uint32_t BATCH_ITEM_WORD = 0;  // LDS word of the item slot (inside the tile planes)
