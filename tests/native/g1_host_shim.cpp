@@ -47,9 +47,9 @@ void shim_g1_add_mixed(const uint64_t* p, const uint64_t* qa, int neg, uint64_t*
 // The MSM bucket sum from the identity over n affine bases (8 words each,
 // x = y = 0 the identity), base i negated when neg[i]; out: Jacobian words
 // (all zero for the identity). trace (optional): after each step the
-// accumulator's raw limbs X, Y, Z (27 u32) and its identity flag (1 u32).
+// accumulator's raw limbs X, Y, ZZ, ZZZ (36 u32) and its identity flag (1 u32).
 void shim_g1_bucket_sum(const uint64_t* bases, const int* neg, int n, uint64_t* out, uint32_t* trace) {
-  g1j acc = g1_identity();
+  g1xyzz acc;
   bool inf = true;
   for (int k = 0; k < n; ++k) {
     const uint64_t* qa = bases + 8 * k;
@@ -61,13 +61,16 @@ void shim_g1_bucket_sum(const uint64_t* bases, const int* neg, int n, uint64_t* 
     a.inf = !any;
     g1_bucket_add(acc, inf, a, neg[k] != 0);
     if (trace) {
-      memcpy(trace + 28 * k, acc.x.l, 36);
-      memcpy(trace + 28 * k + 9, acc.y.l, 36);
-      memcpy(trace + 28 * k + 18, acc.z.l, 36);
-      trace[28 * k + 27] = inf ? 1u : 0u;
+      memset(trace + 37 * k, 0, 37 * 4);
+      if (!inf) {
+        memcpy(trace + 37 * k, acc.x.l, 36);
+        memcpy(trace + 37 * k + 9, acc.y.l, 36);
+        memcpy(trace + 37 * k + 18, acc.zz.l, 36);
+        memcpy(trace + 37 * k + 27, acc.zzz.l, 36);
+      }
+      trace[37 * k + 36] = inf ? 1u : 0u;
     }
   }
-  if (inf) acc = g1_identity();
-  jstore(out, acc);
+  jstore(out, inf ? g1_identity() : g1_xyzz_to_jacobian(acc));
 }
 }

@@ -120,9 +120,9 @@ def _bases_words(points):
 def _bucket_sum(shim, points, negs):
     n = len(points)
     out = U64x12()
-    trace = (ctypes.c_uint32 * (28 * n))()
+    trace = (ctypes.c_uint32 * (37 * n))()
     shim.shim_g1_bucket_sum(_bases_words(points), (ctypes.c_int * n)(*negs), n, out, trace)
-    return aff(out), np.frombuffer(trace, dtype=np.uint32).reshape(n, 28)
+    return aff(out), np.frombuffer(trace, dtype=np.uint32).reshape(n, 37)
 
 
 def _limb_value(l):
@@ -134,8 +134,10 @@ def test_bucket_sum(shim):
     affine bases with signs, against affine Python arithmetic, through the
     equal-x branches (a base twice in a row: doubling; a base then its
     negation: the identity, after which the sum starts again), identity bases
-    and long runs. After every step the accumulator holds its invariant:
-    X, Y, Z normalized (limbs 0..7 < 2^29) with values below 3q."""
+    and long runs. The accumulator is in XYZZ coordinates; after every step
+    it holds its invariant: X, Y, ZZ, ZZZ normalized (limbs 0..7 < 2^29) with
+    values below 3q (ZZ and ZZZ, multiplier outputs, below 1.07q), it stands
+    for the affine sum so far (x = X / ZZ, y = Y / ZZZ), and ZZ^3 = ZZZ^2."""
     rnd = random.Random(7)
     R = O.R_MOD
     pool = [aff_mul(rnd.randrange(1, R), G) for _ in range(24)]
@@ -158,10 +160,21 @@ def test_bucket_sum(shim):
                 want = aff_add(want, aff_neg(q) if ng else q)
         got, trace = _bucket_sum(shim, pts, negs)
         assert got == want, (len(pts), negs[:6])
-        for row in trace:
-            if row[27]:
+        rinv = pow(1 << 261, -1, Q)  # limbs hold v * 2^261 mod q
+        part = None
+        for q, ng, row in zip(pts, negs, trace):
+            if q is not None:
+                part = aff_add(part, aff_neg(q) if ng else q)
+            if row[36]:
+                assert part is None
                 continue
-            for c in range(3):
+            vals = []
+            for c in range(4):
                 l = row[9 * c: 9 * c + 9]
                 assert all(int(v) < (1 << 29) for v in l[:8])
                 assert _limb_value(l) < 3 * Q
+                vals.append(_limb_value(l) * rinv % Q)
+            X, Y, ZZ, ZZZ = vals
+            assert _limb_value(row[18:27]) < 1.07 * Q and _limb_value(row[27:36]) < 1.07 * Q
+            assert pow(ZZ, 3, Q) == pow(ZZZ, 2, Q)
+            assert (X * pow(ZZ, -1, Q) % Q, Y * pow(ZZZ, -1, Q) % Q) == part

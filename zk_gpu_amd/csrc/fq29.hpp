@@ -293,23 +293,61 @@ ZK_HD bool q_is_zero3(const fe& a) {
 }
 }  // namespace fq
 
+// The MSM's bucket accumulator in XYZZ coordinates: x = X / ZZ,
+// y = Y / ZZZ with ZZ^3 = ZZZ^2 (ZZ = Z^2, ZZZ = Z^3 of a Jacobian Z). A
+// mixed addition costs 8M + 2S there (madd-2008-s), against 7M + 4S in
+// Jacobian coordinates.
+struct g1xyzz {
+  fe x, y, zz, zzz;
+};
+
+// Jacobian form of a (not the identity): (X ZZ^2, Y ZZZ^2, ZZZ), since
+// ZZZ^2 = ZZ^3 gives X ZZ^2 / ZZZ^2 = X / ZZ and Y ZZZ^2 / ZZZ^3 = Y / ZZZ.
+ZK_HD g1j g1_xyzz_to_jacobian(const g1xyzz& a) {
+  using namespace fq;
+  return g1j{q_mul(a.x, q_mul(a.zz, a.zz)), q_mul(a.y, q_mul(a.zzz, a.zzz)), a.zzz};
+}
+
+// 2A in XYZZ coordinates, dbl-2008-s-1 (a = 0): 6M + 3S. Bounds in units of
+// q as for g1_bucket_add: U = 2 Y1 < 6, V = U^2 < 1.22, W = U V < 1.05,
+// S = X1 V < 1.03, M = 3 X1^2 < 3.2, M^2 < 1.07, X3 = M^2 + 6q - 2S < 7.07,
+// S + 9q - X3 < 10.03, M (S - X3) < 1.19, W Y1 < 1.02, Y3 < 4.19; X3 and Y3
+// value-reduced below 3q. BN256 G1 has no point of order two, so Y1 != 0.
+ZK_HD void g1_xyzz_double(g1xyzz& a) {
+  using namespace fq;
+  const fe U = q_norm(zkgpu::add(a.y, a.y));
+  const fe V = q_mul(U, U);
+  const fe W = q_mul(U, V);
+  const fe S = q_mul(a.x, V);
+  const fe XX = q_mul(a.x, a.x);
+  const fe M = q_norm(zkgpu::add(zkgpu::add(XX, XX), XX));
+  fe X3 = q_norm(q_sub_lazy<3>(q_sub_lazy<3>(q_mul(M, M), S), S));
+  fe Y3 = q_norm(q_sub_lazy<3>(q_mul(q_sub_lazy<9>(S, X3), M), q_mul(W, a.y)));
+  a.zz = q_mul(V, a.zz);
+  a.zzz = q_mul(W, a.zzz);
+  q_reduce(X3);
+  q_reduce(Y3);
+  a.x = X3;
+  a.y = Y3;
+}
+
 // The MSM's bucket sum: acc += (neg ? -q : q), with the identity carried in
-// *inf (acc is meaningful only when !*inf). Invariant between calls: acc's
-// X, Y, Z normalized with value < 3q. madd-2007-bl with Z3 = 2 Z1 H (one
-// product instead of a square and two subtractions) and lazy field steps;
-// bounds in units of q, from mul's V_out < 1 + V_a V_b q / 2^261 with
-// q / 2^261 < 1/169 (operand a: limbs <= MUL_A_LIMB_MAX; b: normalized):
+// *inf (acc is meaningful only when !*inf). Invariant between calls: X, Y
+// normalized with value < 3q; ZZ, ZZZ normalized products (value < 1.07q).
+// madd-2008-s with lazy field steps; bounds in units of q, from mul's
+// V_out < 1 + V_a V_b q / 2^261 with q / 2^261 < 1/169 (operand a: limbs
+// <= MUL_A_LIMB_MAX; b: normalized):
 //   y2 (q.y or 3q - q.y, limbs < 2^30)  < 3
-//   Z1Z1 < 1.06, Z1 Z1Z1 < 1.02, U2 < 1.01, S2 = y2 Z1 Z1Z1 < 1.02
-//   H = U2 + 4q - X1 (normalized)  < 5.02;  R = 2 (S2 + 4q - Y1) (normalized) < 10.04
-//   HH < 1.15, I = 4 HH (normalized) < 4.61, J = H I < 1.14, V = X1 I < 1.09, R^2 < 1.61
-//   X3 = R^2 + 3q - J + 3q - V + 3q - V (normalized) < 10.61
-//   V + 12q - X3 (limbs < 2^29 + 2^30) < 13.09, R (V - X3) < 1.78, Y1 J < 1.03
-//   Y3 = R (V - X3) + 3q - Y1 J + 3q - Y1 J < 7.8;  Z3 = 2 Z1 H < 2.19
-// X3 and Y3 are normalized and value-reduced (below 3q), Z3 normalized.
-// Z1 != 0, so Z3 == 0 exactly when H == 0: then P = +-Q, and the sum is 2P
+//   U2 = x2 ZZ1 < 1.01, S2 = y2 ZZZ1 < 1.02
+//   P = U2 + 4q - X1 (normalized) < 5.01;  R = S2 + 4q - Y1 (normalized) < 5.02
+//   PP < 1.15, PPP = P PP < 1.04, Q = X1 PP < 1.03, R^2 < 1.15
+//   X3 = R^2 + 3q - PPP + 3q - Q + 3q - Q (normalized) < 10.15
+//   Q + 12q - X3 (limbs < 2^29 + 2^30) < 13.03, R (Q - X3) < 1.39, Y1 PPP < 1.02
+//   Y3 = R (Q - X3) + 3q - Y1 PPP < 4.39;  ZZ3 = ZZ1 PP, ZZZ3 = ZZZ1 PPP < 1.01
+// X3 and Y3 are normalized and value-reduced (below 3q). ZZ1 != 0, so ZZ3
+// == 0 exactly when P == 0: then the base equals +-acc, and the sum is 2 acc
 // (R == 0) or the identity.
-ZK_HD void g1_bucket_add(g1j& acc, bool& inf, const g1a& q, bool neg) {
+ZK_HD void g1_bucket_add(g1xyzz& acc, bool& inf, const g1a& q, bool neg) {
   using namespace fq;
   if (q.inf) return;
   const fe yn = q_sub_lazy<3>(q_zero(), q.y);
@@ -319,39 +357,32 @@ ZK_HD void g1_bucket_add(g1j& acc, bool& inf, const g1a& q, bool neg) {
   if (inf) {
     acc.x = q.x;
     acc.y = q_norm(y2);
-    acc.z = q_from_const(fq::ONE261);
+    acc.zz = q_from_const(fq::ONE261);
+    acc.zzz = acc.zz;
     inf = false;
     return;
   }
-  const fe Z1Z1 = q_mul(acc.z, acc.z);
-  const fe U2 = q_mul(q.x, Z1Z1);
-  const fe S2 = q_mul(y2, q_mul(acc.z, Z1Z1));
-  const fe H = q_norm(q_sub_lazy<4>(U2, acc.x));
-  const fe d = q_sub_lazy<4>(S2, acc.y);
-  const fe R = q_norm(zkgpu::add(d, d));
-  const fe HH = q_mul(H, H);
-  fe I = HH;
-#pragma unroll
-  for (int i = 0; i < NLIMBS; ++i) I.l[i] <<= 2;
-  normalize(I);
-  const fe J = q_mul(H, I);
-  const fe V = q_mul(acc.x, I);
-  const fe X3 = q_norm(q_sub_lazy<3>(q_sub_lazy<3>(q_sub_lazy<3>(q_mul(R, R), J), V), V));
-  const fe YJ = q_mul(acc.y, J);
-  fe Y3 = q_norm(q_sub_lazy<3>(q_sub_lazy<3>(q_mul(q_sub_lazy<12>(V, X3), R), YJ), YJ));
-  const fe ZH = q_mul(acc.z, H);
-  const fe Z3 = q_norm(zkgpu::add(ZH, ZH));
-  if (q_is_zero3(Z3)) {
-    if (q_is_zero(R)) acc = g1_double(acc);
+  const fe U2 = q_mul(q.x, acc.zz);
+  const fe S2 = q_mul(y2, acc.zzz);
+  const fe P = q_norm(q_sub_lazy<4>(U2, acc.x));
+  const fe R = q_norm(q_sub_lazy<4>(S2, acc.y));
+  const fe PP = q_mul(P, P);
+  const fe PPP = q_mul(P, PP);
+  const fe ZZ3 = q_mul(acc.zz, PP);
+  if (q_is_zero3(ZZ3)) {
+    if (q_is_zero(R)) g1_xyzz_double(acc);
     else inf = true;
     return;
   }
-  fe X = X3;
-  q_reduce(X);
+  const fe Q = q_mul(acc.x, PP);
+  fe X3 = q_norm(q_sub_lazy<3>(q_sub_lazy<3>(q_sub_lazy<3>(q_mul(R, R), PPP), Q), Q));
+  fe Y3 = q_norm(q_sub_lazy<3>(q_mul(q_sub_lazy<12>(Q, X3), R), q_mul(acc.y, PPP)));
+  acc.zzz = q_mul(acc.zzz, PPP);
+  acc.zz = ZZ3;
+  q_reduce(X3);
   q_reduce(Y3);
-  acc.x = X;
+  acc.x = X3;
   acc.y = Y3;
-  acc.z = Z3;
 }
 
 // P + Q, add-2007-bl: 11M + 5S

@@ -12,7 +12,8 @@
 //                 (one 16-bit key per window)
 //   sort          the (window, point) pairs into bucket order: two radix
 //                 passes through LDS (partition scan, partition, bucket sort)
-//   msm_buckets   one thread per bucket: sum of its points (mixed additions)
+//   msm_buckets   one thread per bucket: sum of its points (mixed additions
+//                 in XYZZ coordinates)
 //   msm_chunks    one thread per run of L buckets of a window: running sums
 //                 give sum_j (j+1) B_j over the run
 //   msm_windows   one workgroup per window: tree sum of its runs, stored as
@@ -328,14 +329,15 @@ __global__ void __launch_bounds__(SORT_THREADS) msm_bucket_sort_kernel(
   }
 }
 
-// One thread per bucket: the sum of its points.
+// One thread per bucket: the sum of its points, accumulated in XYZZ
+// coordinates and stored in Jacobian ones.
 __global__ void msm_buckets_kernel(const BaseLimbs* __restrict__ blimbs, const uint32_t* __restrict__ offsets,
                                    const uint32_t* __restrict__ counts, const uint32_t* __restrict__ list,
                                    uint32_t nbuckets, PointLimbs* __restrict__ buckets) {
   const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nbuckets) return;
   const uint32_t start = offsets[b], cnt = counts[b];
-  g1j acc = g1_identity();
+  g1xyzz acc;
   bool inf = true;
   // software pipeline: the list entry two ahead and the base one ahead are
   // in flight while the current addition runs
@@ -352,8 +354,7 @@ __global__ void msm_buckets_kernel(const BaseLimbs* __restrict__ blimbs, const u
     }
     g1_bucket_add(acc, inf, cur, neg);
   }
-  if (inf) acc = g1_identity();
-  store_point(buckets, b, acc);
+  store_point(buckets, b, inf ? g1_identity() : g1_xyzz_to_jacobian(acc));
 }
 
 // One thread per run of `run` buckets [lo, lo + run) of window w:
