@@ -30,6 +30,8 @@ PEAK_HBM_GBS = 8000.0
 # 256 CUs x 64 lanes x 2.4 GHz; v_mad_u64_u32 issues at the full VALU rate on gfx950
 PEAK_MAD_TOPS = 256 * 64 * 2.4e9 / 1e12
 MADS_PER_MODMUL = 136
+# Fq products of one mixed addition in the MSM's bucket sums (XYZZ, 8M + 2S)
+FQ_PRODUCTS_PER_MADD = 10
 
 
 def modmuls(log_n, op):
@@ -306,8 +308,8 @@ def bench_msm(torch, Z, dev, stream, p_mul, log_ns=(20, 24)):
     Fr scalars, bases k_i G. Wall time per call (the call returns the result
     on the host: launches, window sums, host Horner); kernel durations from
     events around each MSM launch. The bucket kernel's algorithmic work is
-    n * W mixed additions (Pippenger, W windows of c bits), each 11 Fq products
-    (madd-2007-bl: 7M + 4S) of 136 textbook multiply-adds."""
+    n * W mixed additions (Pippenger, W windows of c bits), each 10 Fq products
+    (madd-2008-s in XYZZ coordinates: 8M + 2S) of 136 textbook multiply-adds."""
     import ctypes
     sptr = stream.cuda_stream
     out = []
@@ -335,7 +337,7 @@ def bench_msm(torch, Z, dev, stream, p_mul, log_ns=(20, 24)):
         Z.pass_timing(False)
         kernels = {k: v[1] / v[0] * 1e3 for k, v in kt.items()}  # us per launch
         madds = n * windows
-        mads = madds * 11 * MADS_PER_MODMUL
+        mads = madds * FQ_PRODUCTS_PER_MADD * MADS_PER_MODMUL
         tb = kernels.get("msm_buckets_kernel", 0) * 1e-6
         ach = mads / tb / 1e12 if tb > 0 else None
         out.append({"log_n": log_n, "points": n, "window_bits": c, "windows": windows, "workspace_bytes": ws,
@@ -344,7 +346,7 @@ def bench_msm(torch, Z, dev, stream, p_mul, log_ns=(20, 24)):
                                  "peak": PEAK_MAD_TOPS, "unit": "Tmad/s",
                                  "frac": ach / PEAK_MAD_TOPS if ach else None,
                                  "frac_measured": ach / p_mul if ach else None,
-                                 "work": f"{madds} mixed additions x 11 Fq products x 136 multiply-adds"}})
+                                 "work": f"{madds} mixed additions x {FQ_PRODUCTS_PER_MADD} Fq products x 136 multiply-adds"}})
         del sc, bs
     res = {"config": "G1 MSM (BN256, y^2 = x^3 + 3), random Fr scalars, synthetic bases k_i G, device resident",
            "sizes": out, "reference_logged_k24_s": REF_MSM_K24_S,

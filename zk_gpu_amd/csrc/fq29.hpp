@@ -16,7 +16,8 @@
 // has Z = 0) follow the same textbook formulas the reference uses
 // (cuda/curve.cuh:106-297): doubling dbl-2009-l, mixed addition
 // madd-2007-bl, general addition add-2007-bl, with the equal-x cases routed
-// to doubling or the identity.
+// to doubling or the identity. The MSM's bucket sums accumulate in XYZZ
+// coordinates instead (madd-2008-s, dbl-2008-s-1; g1_bucket_add).
 #pragma once
 #include <stdint.h>
 
