@@ -892,7 +892,9 @@ int Engine::run(const JobSpec& job, hipStream_t s, void* planar_ws) {
       TimedPass tp;
       tp.a = a;
       tp.b = b;
-      tp.name = std::string(pass_kernel_narrow(descs[i]) ? "ntt_pass_kernel_narrow<" : "ntt_pass_kernel<") +
+      tp.name = std::string(pass_kernel_narrow(descs[i]) ? "ntt_pass_kernel_narrow<"
+                            : pass_kernel_quad(descs[i]) ? "ntt_pass_kernel_quad<"
+                                                         : "ntt_pass_kernel<") +
                 std::to_string(descs[i].log_n_tile) + ">";
       timed_.push_back(tp);
     }

@@ -331,6 +331,7 @@ hipError_t configure_kernels();
 // start/stop (optional): events tied to the dispatch (hipExtLaunchKernel)
 hipError_t launch_pass(const PassDesc& d, hipStream_t s, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 bool pass_kernel_narrow(const PassDesc& d);  // the narrow pass-kernel family runs d
+bool pass_kernel_quad(const PassDesc& d);    // the quad (four elements per thread) family runs d
 hipError_t launch_pow_table(void* out, const uint32_t base[8], uint64_t step, uint32_t count, bool limbs48,
                             hipStream_t s);
 hipError_t launch_pow_table_2level(void* out, const void* lo, const void* hi, uint32_t h, uint64_t count,

@@ -340,7 +340,8 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n) {
 
 constexpr int MAX_TILE_THREADS = 1 << (TILE_LOG_MAX - 3);
 
-// Single-pass tiles in two families chosen at launch:
+// Single-pass tiles in three families chosen at launch (quad, below, is the
+// third):
 //   narrow: tiles up to 2^11 elements (<= 256 threads) in passes with more
 //     than two waves per SIMD: two-phase exchange, at most 128 VGPRs, four
 //     waves per SIMD (four 40 KiB tiles fill a CU's 160 KiB), single-chain
@@ -372,6 +373,19 @@ template <int LOGN>
 __global__ void __launch_bounds__(1 << (NARROW_TILE_LOG - 3), 4) ntt_pass_kernel_narrow(PassDesc d) {
   extern __shared__ uint32_t lds[];
   ntt_tile<LOGN, NarrowXchg>(d, xcd_tile(blockIdx.x, d.num_tiles), lds, threadIdx.x, true);
+}
+// Quad: the wide configuration with four elements per thread (radix-4
+// rounds, a radix-2 round for an odd digit count), for passes that eight
+// elements per thread would give fewer than two waves per SIMD (transforms
+// below 2^20 elements): twice the waves for more LDS rounds (a 2^19
+// transform 0.060 instead of 0.078 ms; at 2^20, with two waves per SIMD
+// either way, the extra rounds cost more than the waves gain). 70-78 VGPRs;
+// a 2^12 tile is one 1024-thread workgroup.
+using QuadXchg = Xchg<false, true, 2, true>;
+template <int LOGN>
+__global__ void __launch_bounds__(1024, 4) ntt_pass_kernel_quad(PassDesc d) {
+  extern __shared__ uint32_t lds[];
+  ntt_tile<LOGN, QuadXchg>(d, xcd_tile(blockIdx.x, d.num_tiles), lds, threadIdx.x, true);
 }
 
 // Persistent pass kernel for batches: a grid of resident workgroups drains a
@@ -604,6 +618,16 @@ static bool use_narrow(const PassDesc& d) {
   return false;
 }
 
+// Passes left to the wide configuration that eight elements per thread would
+// give fewer than two waves per SIMD run the quad configuration.
+template <int LOGN>
+static bool use_quad(const PassDesc& d) {
+  if (use_narrow<LOGN>(d)) return false;
+  const uint32_t threads = tile_threads(LOGN, d.log_c, PassXchg::le_max);
+  const uint64_t waves = ((uint64_t)d.num_tiles * threads + 63) / 64;
+  return waves < g_narrow_min_waves.load(std::memory_order_relaxed);
+}
+
 // <<<>>> when no event is asked for; otherwise hipExtLaunchKernel, which ties
 // the events to the dispatch itself (no separate event packet on the stream).
 template <class K>
@@ -627,6 +651,10 @@ static hipError_t launch_one(const PassDesc& d, hipStream_t s, hipEvent_t start,
                            (4u * NarrowXchg::words) << tl, s, d, start, stop);
     }
   }
+  if (use_quad<LOGN>(d)) {
+    return launch_kernel(ntt_pass_kernel_quad<LOGN>, d.num_tiles, tile_threads(LOGN, d.log_c, QuadXchg::le_max),
+                         (4u * QuadXchg::words) << tl, s, d, start, stop);
+  }
   return launch_kernel(ntt_pass_kernel<LOGN>, d.num_tiles, tile_threads(LOGN, d.log_c, PassXchg::le_max),
                        (4u * PassXchg::words) << tl, s, d, start, stop);
 }
@@ -638,6 +666,15 @@ hipError_t launch_pass(const PassDesc& d, hipStream_t s, hipEvent_t start, hipEv
     ZK_TILE_CASES(X)
 #undef X
     default: return hipErrorInvalidValue;
+  }
+}
+
+bool pass_kernel_quad(const PassDesc& d) {
+  switch (d.log_n_tile) {
+#define X(L) case L: return use_quad<L>(d);
+    ZK_TILE_CASES(X)
+#undef X
+    default: return false;
   }
 }
 
@@ -736,6 +773,9 @@ hipError_t configure_kernels() {
   if (e == hipSuccess)                                                                                    \
     e = hipFuncSetAttribute((const void*)ntt_pass_kernel<L>, hipFuncAttributeMaxDynamicSharedMemorySize, \
                             max_lds);                                                                     \
+  if (e == hipSuccess)                                                                                    \
+    e = hipFuncSetAttribute((const void*)ntt_pass_kernel_quad<L>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                            (int)(4u * QuadXchg::words) << TILE_LOG_MAX);                                 \
   if (e == hipSuccess && L <= (int)NARROW_TILE_LOG)                                                       \
     e = hipFuncSetAttribute((const void*)ntt_pass_kernel_narrow<(L <= 11 ? L : 11)>,                      \
                             hipFuncAttributeMaxDynamicSharedMemorySize,                                   \
