@@ -93,6 +93,35 @@ def test_closed_form_large(log_n):
     assert _msm_host(scalars, bases) == want
 
 
+def _skewed(kind, n, rnd):
+    """scalars whose bucket counts are far from uniform: one bucket of a
+    window then holds most of the n points, and its sum is split over many
+    threads"""
+    if kind == "equal":
+        return np.tile(O.elem(rnd.randrange(R)), (n, 1))
+    if kind == "zero_one":
+        bits = np.array([rnd.getrandbits(1) for _ in range(n)], dtype=np.uint64)
+        return np.ascontiguousarray(np.where(bits[:, None] == 1, O.elem(1), np.uint64(0)))
+    if kind == "minus_one":
+        return np.tile(O.elem(R - 1), (n, 1))
+    if kind == "small":  # canonical values below 2^12, the rest of every window zero
+        return np.ascontiguousarray(np.concatenate([O.elem(rnd.randrange(1 << 12)) for _ in range(n)]))
+    raise ValueError(kind)
+
+
+@pytest.mark.parametrize("kind,log_n", [("equal", 18), ("zero_one", 18), ("minus_one", 16), ("small", 16),
+                                        ("equal", 20), ("zero_one", 12), ("equal", 22), ("zero_one", 22)])
+def test_skewed_scalars(kind, log_n):
+    """closed form over scalar sets that put most points into a few buckets
+    (at 2^22 the ordinary segments are longer than a heavy bucket's)"""
+    n = 1 << log_n
+    rnd = random.Random(log_n * 31 + len(kind))
+    a, d = rnd.randrange(1, R), rnd.randrange(1, R)
+    bases = O.g1_arith_bases(n, a, d)
+    scalars = _skewed(kind, n, rnd)
+    assert _msm_host(scalars, bases) == O.g1_mul_gen(_weighted_sum(scalars, a, d)), kind
+
+
 def test_device_entry_point(torch_cuda):
     torch = torch_cuda
     n = 5000

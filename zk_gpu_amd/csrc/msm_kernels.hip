@@ -106,13 +106,30 @@ __device__ __forceinline__ fe fr_from_mont_factor() {
 //   msm_partition_kernel    per 4096-pair chunk: local sort by partition,
 //                           one atomic per partition reserves the chunk's run
 //   msm_bucket_sort_kernel  per partition: bucket counts and offsets, then
-//                           the entries i | sign << 31 in bucket order
+//                           the entries i | sign << 31 in bucket order; and
+//                           each bucket's segments (runs of its entries, the
+//                           bucket sums' unit of work)
 constexpr uint32_t MSM_KEY_NONE = 0xFFFFu;
 constexpr uint32_t MSM_MAX_PARTS = 128;           // 2^(15 - 8): c <= 16
 constexpr uint32_t MSM_MAX_COUNTERS = 2048;       // windows x partitions, any c in 4..16
 constexpr uint32_t SORT_THREADS = 256;
 constexpr uint32_t SORT_PER_THREAD = 16;
 constexpr uint32_t SORT_CHUNK = SORT_THREADS * SORT_PER_THREAD;
+// A bucket segment is a run of entries of one bucket, the bucket sums' unit
+// of work: a bucket of more entries (skewed scalars, a sparse top window) is
+// summed by several threads and their partial sums merged. Segments are
+// 2^seg_log entries long (chosen by msm_seg_log so that ordinary buckets are
+// one segment); a heavy bucket, of more than two such segments, is cut into
+// segments of 2^MSM_HEAVY_SEG_LOG entries instead, so that a few heavy buckets
+// still give enough threads to fill the device.
+constexpr uint32_t MSM_HEAVY_SEG_LOG = 8;  // <= every seg_log (msm_seg_log's floor)
+__device__ __forceinline__ uint32_t bucket_seg_log(uint32_t count, uint32_t seg_log) {
+  return count > (2u << seg_log) ? MSM_HEAVY_SEG_LOG : seg_log;
+}
+__device__ __forceinline__ uint32_t bucket_segs(uint32_t count, uint32_t seg_log) {
+  const uint32_t lg = bucket_seg_log(count, seg_log);
+  return (count + (1u << lg) - 1) >> lg;
+}
 
 // Per point: base in limb form; signed digits of the scalar as keys
 // (keys[w * n + i]); partition sizes part_counts[w * parts + hi] (zeroed by
@@ -172,11 +189,12 @@ __global__ void __launch_bounds__(256) msm_digits_kernel(const uint4* __restrict
 }
 
 // Exclusive scan of the m <= MSM_MAX_COUNTERS partition sizes (window-major)
-// into part_off, and the same into part_cursor (the partition kernel's
-// reservation cursors). One workgroup of 256.
+// into part_off[0..m] (part_off[m] = the total), and the same into
+// part_cursor (the partition kernel's reservation cursors; the two may be
+// one array). One workgroup of 256. Also scans the partitions' segment
+// counts.
 __global__ void __launch_bounds__(256) msm_part_scan_kernel(const uint32_t* __restrict__ part_counts, uint32_t m,
-                                                            uint32_t* __restrict__ part_off,
-                                                            uint32_t* __restrict__ part_cursor) {
+                                                            uint32_t* part_off, uint32_t* part_cursor) {
   __shared__ uint32_t part[256];
   const uint32_t per = (m + 255) / 256;
   const uint32_t lo = threadIdx.x * per, hi = min(m, lo + per);
@@ -198,6 +216,10 @@ __global__ void __launch_bounds__(256) msm_part_scan_kernel(const uint32_t* __re
     part_off[k] = run;
     part_cursor[k] = run;
     run += part_counts[k];
+  }
+  if (threadIdx.x == 255) {  // its range ends at m: run is the total
+    part_off[m] = run;
+    part_cursor[m] = run;
   }
 }
 
@@ -260,7 +282,9 @@ __global__ void __launch_bounds__(SORT_THREADS) msm_partition_kernel(const uint1
 __global__ void __launch_bounds__(SORT_THREADS) msm_bucket_sort_kernel(
     const uint32_t* __restrict__ part_off, const uint32_t* __restrict__ part_counts,
     const uint32_t* __restrict__ t_entry, const uint8_t* __restrict__ t_lo, uint32_t c, uint32_t lo_bits,
-    uint32_t* __restrict__ counts, uint32_t* __restrict__ offsets, uint32_t* __restrict__ list) {
+    uint32_t* __restrict__ counts, uint32_t* __restrict__ offsets, uint32_t* __restrict__ list,
+    uint32_t seg_log, uint32_t* __restrict__ seg_local, uint32_t* __restrict__ part_segs,
+    uint32_t* __restrict__ seg_max) {
   __shared__ uint32_t cur[256], lcnt[256], loff[256];
   __shared__ uint32_t st_entry[SORT_CHUNK];
   __shared__ uint8_t st_lo[SORT_CHUNK];
@@ -273,11 +297,17 @@ __global__ void __launch_bounds__(SORT_THREADS) msm_bucket_sort_kernel(
   for (uint32_t j = threadIdx.x; j < m; j += SORT_THREADS) atomicAdd(&lcnt[t_lo[start + j]], 1u);
   __syncthreads();
   if (threadIdx.x == 0) {
-    uint32_t run = start;
+    uint32_t run = start, segs = 0, most = 0;
     for (uint32_t l = 0; l < nlo; ++l) {
       cur[l] = run;
       run += lcnt[l];
+      loff[l] = segs;  // the bucket's first segment within the partition
+      const uint32_t bs = bucket_segs(lcnt[l], seg_log);
+      segs += bs;
+      most = max(most, bs);
     }
+    part_segs[g] = segs;
+    if (most > 1) atomicMax(seg_max, most);
   }
   __syncthreads();
   const uint32_t half = 1u << (c - 1);
@@ -285,6 +315,7 @@ __global__ void __launch_bounds__(SORT_THREADS) msm_bucket_sort_kernel(
     const uint32_t b = w * half + (p << lo_bits) + l;
     counts[b] = lcnt[l];
     offsets[b] = cur[l];
+    seg_local[b] = loff[l];
   }
   for (uint32_t c0 = 0; c0 < m; c0 += SORT_CHUNK) {
     const uint32_t len = min(SORT_CHUNK, m - c0);
@@ -329,14 +360,34 @@ __global__ void __launch_bounds__(SORT_THREADS) msm_bucket_sort_kernel(
   }
 }
 
-// One thread per bucket: the sum of its points, accumulated in XYZZ
-// coordinates and stored in Jacobian ones.
+// Global index of bucket b's first segment: the segment offset of its
+// partition (b >> lo_bits) plus its offset inside the partition.
+__device__ __forceinline__ uint32_t seg_prefix(const uint32_t* part_seg_off, const uint32_t* seg_local, uint32_t b,
+                                               uint32_t lo_bits) {
+  return part_seg_off[b >> lo_bits] + seg_local[b];
+}
+
+// One thread per bucket segment (a run of one bucket's entries): the
+// sum of its points, accumulated in XYZZ coordinates, stored in Jacobian ones.
+// The segment's bucket is the largest b whose first segment is <= s (empty
+// buckets have no segments, so that bucket is never empty).
 __global__ void msm_buckets_kernel(const BaseLimbs* __restrict__ blimbs, const uint32_t* __restrict__ offsets,
                                    const uint32_t* __restrict__ counts, const uint32_t* __restrict__ list,
-                                   uint32_t nbuckets, PointLimbs* __restrict__ buckets) {
-  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nbuckets) return;
-  const uint32_t start = offsets[b], cnt = counts[b];
+                                   const uint32_t* __restrict__ part_seg_off, const uint32_t* __restrict__ seg_local,
+                                   uint32_t nparts, uint32_t lo_bits, uint32_t seg_log, uint32_t nbuckets,
+                                   PointLimbs* __restrict__ seg_sums, uint32_t* __restrict__ seg_bucket) {
+  const uint32_t sidx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (sidx >= part_seg_off[nparts]) return;
+  uint32_t lo = 0, hi = nbuckets;  // largest b with seg_prefix(b) <= sidx
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (seg_prefix(part_seg_off, seg_local, mid, lo_bits) <= sidx) lo = mid; else hi = mid;
+  }
+  const uint32_t b = lo;
+  seg_bucket[sidx] = b;
+  const uint32_t count = counts[b], lg = bucket_seg_log(count, seg_log);
+  const uint32_t first = (sidx - seg_prefix(part_seg_off, seg_local, b, lo_bits)) << lg;
+  const uint32_t start = offsets[b] + first, cnt = min(1u << lg, count - first);
   g1xyzz acc;
   bool inf = true;
   // software pipeline: the list entry two ahead and the base one ahead are
@@ -354,7 +405,43 @@ __global__ void msm_buckets_kernel(const BaseLimbs* __restrict__ blimbs, const u
     }
     g1_bucket_add(acc, inf, cur, neg);
   }
-  store_point(buckets, b, inf ? g1_identity() : g1_xyzz_to_jacobian(acc));
+  store_point(seg_sums, sidx, inf ? g1_identity() : g1_xyzz_to_jacobian(acc));
+}
+
+// One level of the pairwise reduction of each bucket's segment sums: the
+// segment at local index i (a multiple of 2 x 2^level) adds the one at
+// i + 2^level, if the bucket has it. After the levels 0..L with 2^(L+1) >=
+// the most segments of any bucket, a bucket's first segment holds its sum.
+// A level whose stride 2^level is at least seg_max (the most segments of one
+// bucket, from the sort) exits at once: the host launches the levels a bucket
+// of n entries could need. (Eight to one per level took fewer launches but
+// lengthened the chains of the active levels: slower with skewed scalars.)
+__global__ void __launch_bounds__(256) msm_segment_tree_kernel(
+    PointLimbs* __restrict__ seg_sums, const uint32_t* __restrict__ seg_bucket, const uint32_t* __restrict__ counts,
+    const uint32_t* __restrict__ part_seg_off, const uint32_t* __restrict__ seg_local,
+    const uint32_t* __restrict__ seg_max, uint32_t nparts, uint32_t lo_bits, uint32_t seg_log, uint32_t level) {
+  const uint32_t step = 1u << level;
+  if (*seg_max <= step) return;
+  const uint32_t total = part_seg_off[nparts];
+  for (uint32_t sidx = blockIdx.x * blockDim.x + threadIdx.x; sidx < total; sidx += gridDim.x * blockDim.x) {
+    const uint32_t b = seg_bucket[sidx];
+    const uint32_t i = sidx - seg_prefix(part_seg_off, seg_local, b, lo_bits);
+    if ((i & (2 * step - 1)) != 0) continue;
+    if (i + step >= bucket_segs(counts[b], seg_log)) continue;
+    store_point(seg_sums, sidx, g1_add(load_point(seg_sums, sidx), load_point(seg_sums, sidx + step)));
+  }
+}
+
+// One thread per bucket: its sum (the first segment's, after the reduction)
+// or the identity for an empty bucket.
+__global__ void msm_bucket_gather_kernel(const PointLimbs* __restrict__ seg_sums, const uint32_t* __restrict__ counts,
+                                         const uint32_t* __restrict__ part_seg_off,
+                                         const uint32_t* __restrict__ seg_local, uint32_t lo_bits, uint32_t nbuckets,
+                                         PointLimbs* __restrict__ buckets) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbuckets) return;
+  store_point(buckets, b,
+              counts[b] ? load_point(seg_sums, seg_prefix(part_seg_off, seg_local, b, lo_bits)) : g1_identity());
 }
 
 // One thread per run of `run` buckets [lo, lo + run) of window w:
@@ -474,7 +561,9 @@ hipError_t launch_msm_digits(const void* scalars, const void* bases, uint32_t n,
 }
 hipError_t launch_msm_sort(const uint16_t* keys, uint32_t n, uint32_t c, uint32_t windows, uint32_t lo_bits,
                            const uint32_t* part_counts, uint32_t* part_off, uint32_t* part_cursor, uint32_t* t_entry,
-                           uint8_t* t_lo, uint32_t* counts, uint32_t* offsets, uint32_t* list, hipStream_t s) {
+                           uint8_t* t_lo, uint32_t* counts, uint32_t* offsets, uint32_t* list, uint32_t seg_log,
+                           uint32_t* seg_local, uint32_t* part_segs, uint32_t* part_seg_off, uint32_t* seg_max,
+                           hipStream_t s) {
   if (c < 2 || c > 16 || lo_bits > 8 || lo_bits > c - 1) return hipErrorInvalidValue;
   const uint32_t parts = 1u << (c - 1 - lo_bits);
   if (parts > MSM_MAX_PARTS || windows * parts > MSM_MAX_COUNTERS) return hipErrorInvalidValue;
@@ -482,14 +571,45 @@ hipError_t launch_msm_sort(const uint16_t* keys, uint32_t n, uint32_t c, uint32_
   msm_partition_kernel<<<dim3((n + SORT_CHUNK - 1) / SORT_CHUNK, windows), SORT_THREADS, 0, s>>>(
       keys, n, c, lo_bits, part_cursor, t_entry, t_lo);
   msm_bucket_sort_kernel<<<windows * parts, SORT_THREADS, 0, s>>>(part_off, part_counts, t_entry, t_lo, c, lo_bits,
-                                                                  counts, offsets, list);
+                                                                  counts, offsets, list, seg_log, seg_local,
+                                                                  part_segs, seg_max);
+  msm_part_scan_kernel<<<1, 256, 0, s>>>(part_segs, windows * parts, part_seg_off, part_seg_off);
   return hipGetLastError();
 }
+// Segment length 2^seg_log: four times the mean bucket size (so that a
+// bucket of uniformly random digits is one segment), within 2^8..2^12.
+uint32_t msm_seg_log(uint64_t pairs, uint64_t nbuckets) {
+  uint32_t lg = 8;
+  while (lg < 12 && (1ull << lg) < 4 * pairs / nbuckets) ++lg;
+  return lg;
+}
+// upper bound of the segments of `pairs` entries over `nbuckets` buckets
+// (a bucket of c entries has at most c / 2^MSM_HEAVY_SEG_LOG + 1)
+uint64_t msm_max_segments(uint64_t pairs, uint64_t nbuckets) { return (pairs >> MSM_HEAVY_SEG_LOG) + nbuckets; }
 uint32_t msm_lo_bits(uint32_t c) { return std::min(8u, c - 1); }
 hipError_t launch_msm_buckets(const void* blimbs, const uint32_t* offsets, const uint32_t* counts,
-                              const uint32_t* list, uint32_t nbuckets, void* buckets, hipStream_t s) {
-  msm_buckets_kernel<<<(nbuckets + 63) / 64, 64, 0, s>>>((const BaseLimbs*)blimbs, offsets, counts, list, nbuckets,
-                                                        (PointLimbs*)buckets);
+                              const uint32_t* list, const uint32_t* part_seg_off, const uint32_t* seg_local,
+                              uint32_t nparts, uint32_t lo_bits, uint32_t seg_log, uint32_t nbuckets,
+                              uint64_t max_segs, void* seg_sums, uint32_t* seg_bucket, hipStream_t s) {
+  msm_buckets_kernel<<<(uint32_t)((max_segs + 63) / 64), 64, 0, s>>>(
+      (const BaseLimbs*)blimbs, offsets, counts, list, part_seg_off, seg_local, nparts, lo_bits, seg_log, nbuckets,
+      (PointLimbs*)seg_sums, seg_bucket);
+  return hipGetLastError();
+}
+// The segment sums reduced to one sum per bucket: the tree levels a bucket
+// of n entries could need (idle ones exit on seg_max), then the gather.
+hipError_t launch_msm_bucket_merge(void* seg_sums, const uint32_t* seg_bucket, const uint32_t* counts,
+                                   const uint32_t* part_seg_off, const uint32_t* seg_local, const uint32_t* seg_max,
+                                   uint32_t nparts, uint32_t lo_bits, uint32_t seg_log, uint32_t n, uint32_t nbuckets,
+                                   uint64_t max_segs, void* buckets, hipStream_t s) {
+  // segments of a bucket of at most n entries: two, or n / 2^8 for a heavy one
+  const uint64_t most = std::max<uint64_t>(2, ((uint64_t)n + (1u << MSM_HEAVY_SEG_LOG) - 1) >> MSM_HEAVY_SEG_LOG);
+  const uint32_t blocks = (uint32_t)std::min<uint64_t>(1024, (max_segs + 255) / 256);
+  for (uint32_t level = 0; (1ull << level) < most; ++level)
+    msm_segment_tree_kernel<<<blocks, 256, 0, s>>>((PointLimbs*)seg_sums, seg_bucket, counts, part_seg_off, seg_local,
+                                                   seg_max, nparts, lo_bits, seg_log, level);
+  msm_bucket_gather_kernel<<<(nbuckets + 63) / 64, 64, 0, s>>>((const PointLimbs*)seg_sums, counts, part_seg_off,
+                                                               seg_local, lo_bits, nbuckets, (PointLimbs*)buckets);
   return hipGetLastError();
 }
 hipError_t launch_msm_chunks(const void* buckets, uint32_t half, uint32_t run, uint32_t runs_per_window,
