@@ -21,10 +21,10 @@ namespace zkgpu {
 hipError_t launch_msm_digits(const void* scalars, const void* bases, uint32_t n, uint32_t c, uint32_t windows,
                              uint32_t lo_bits, void* blimbs, uint16_t* keys, uint32_t* part_counts, hipStream_t s);
 hipError_t launch_msm_sort(const uint16_t* keys, uint32_t n, uint32_t c, uint32_t windows, uint32_t lo_bits,
-                           const uint32_t* part_counts, uint32_t* part_off, uint32_t* part_cursor, uint32_t* t_entry,
-                           uint8_t* t_lo, uint32_t* counts, uint32_t* offsets, uint32_t* list, uint32_t seg_log,
-                           uint32_t* seg_local, uint32_t* part_segs, uint32_t* part_seg_off, uint32_t* seg_max,
-                           hipStream_t s);
+                           const uint32_t* part_counts, uint32_t* part_off, uint32_t* part_cursor,
+                           uint32_t* slice_off, uint32_t* t_entry, uint8_t* t_lo, uint32_t* counts, uint32_t* offsets,
+                           uint32_t* cursor, uint32_t* list, uint32_t seg_log, uint32_t* seg_local,
+                           uint32_t* part_segs, uint32_t* part_seg_off, uint32_t* seg_max, hipStream_t s);
 uint32_t msm_lo_bits(uint32_t c);
 uint32_t msm_seg_log(uint64_t pairs, uint64_t nbuckets);
 uint64_t msm_max_segments(uint64_t pairs, uint64_t nbuckets);
@@ -192,7 +192,8 @@ struct MsmPlan {
   uint32_t seg_log = 0;   // bucket segments: runs of 2^seg_log entries of one bucket (heavy ones: shorter)
   uint64_t max_segs = 0;  // bound on their number
   size_t off_blimbs = 0, off_keys = 0, off_tentry = 0, off_tlo = 0, off_list = 0, off_counts = 0, off_offsets = 0,
-         off_pcounts = 0, off_poff = 0, off_pcursor = 0, off_seg_local = 0, off_psegs = 0, off_pseg_off = 0,
+         off_pcounts = 0, off_poff = 0, off_pcursor = 0, off_slice_off = 0, off_cursor = 0, off_seg_local = 0,
+         off_psegs = 0, off_pseg_off = 0,
          off_seg_sums = 0, off_seg_bucket = 0, off_buckets = 0, off_parts = 0, off_wout = 0, bytes = 0;
 };
 
@@ -227,12 +228,16 @@ MsmPlan msm_plan(uint32_t n) {
   p.off_tentry = take(pairs * 4);
   p.off_tlo = take(pairs);
   p.off_list = take(pairs * 4);
-  p.off_counts = take(nb * 4);
-  p.off_offsets = take(nb * 4);
   const size_t m = (size_t)p.windows * p.sort_parts;  // partitions; the scans write m + 1 words
-  p.off_pcounts = take((m + 1) * 4);  // partition sizes, then the most segments of one bucket
+  // one zeroed region: partition sizes, the most segments of one bucket, the
+  // bucket counts
+  p.off_pcounts = take((m + 1 + nb) * 4);
+  p.off_counts = p.off_pcounts + (m + 1) * 4;
+  p.off_offsets = take(nb * 4);
+  p.off_cursor = take(nb * 4);
   p.off_poff = take((m + 1) * 4);
   p.off_pcursor = take((m + 1) * 4);
+  p.off_slice_off = take((m + 1) * 4);
   p.off_seg_local = take(nb * 4);
   p.off_psegs = take(m * 4);
   p.off_pseg_off = take((m + 1) * 4);
@@ -253,7 +258,7 @@ int msm_enqueue(Engine* eng, const MsmPlan& p, const void* d_scalars, const void
                 hipStream_t s) {
   char* b = (char*)ws;
   const uint32_t nb = p.windows * p.half;
-  uint32_t* counts = (uint32_t*)(b + p.off_counts);  // written for every bucket by the sort
+  uint32_t* counts = (uint32_t*)(b + p.off_counts);  // zeroed with the partition sizes, then counted by the sort
   const bool timing = eng->pass_timing();
   hipError_t e = hipSuccess;
   // one launch (or launch group), bracketed by events while timing is on
@@ -276,15 +281,16 @@ int msm_enqueue(Engine* eng, const MsmPlan& p, const void* d_scalars, const void
   const uint32_t m = p.windows * p.sort_parts;
   uint32_t* seg_max = pcounts + m;
   uint32_t* seg_bucket = (uint32_t*)(b + p.off_seg_bucket);
-  step("reset_counters_kernel", [&] { return launch_reset_counters(pcounts, m + 1, s); });
+  step("reset_counters_kernel", [&] { return launch_reset_counters(pcounts, m + 1 + nb, s); });
   step("msm_digits_kernel", [&] {
     return launch_msm_digits(d_scalars, d_bases, p.n, p.c, p.windows, p.lo_bits, b + p.off_blimbs,
                              (uint16_t*)(b + p.off_keys), pcounts, s);
   });
-  step("msm_sort (3 kernels)", [&] {
+  step("msm_sort (6 kernels)", [&] {
     return launch_msm_sort((const uint16_t*)(b + p.off_keys), p.n, p.c, p.windows, p.lo_bits, pcounts,
-                           (uint32_t*)(b + p.off_poff), (uint32_t*)(b + p.off_pcursor), (uint32_t*)(b + p.off_tentry),
-                           (uint8_t*)(b + p.off_tlo), counts, (uint32_t*)(b + p.off_offsets),
+                           (uint32_t*)(b + p.off_poff), (uint32_t*)(b + p.off_pcursor),
+                           (uint32_t*)(b + p.off_slice_off), (uint32_t*)(b + p.off_tentry), (uint8_t*)(b + p.off_tlo),
+                           counts, (uint32_t*)(b + p.off_offsets), (uint32_t*)(b + p.off_cursor),
                            (uint32_t*)(b + p.off_list), p.seg_log, seg_local, (uint32_t*)(b + p.off_psegs), pseg_off,
                            seg_max, s);
   });

@@ -102,19 +102,29 @@ __device__ __forceinline__ fe fr_from_mont_factor() {
 //   msm_digits_kernel       keys, and per-window partition sizes (LDS
 //                           histograms, one global atomic per counter and
 //                           workgroup)
-//   msm_part_scan_kernel    partition offsets over all windows
+//   msm_part_scan_kernel    partition offsets over all windows, and the
+//                           partitions' slices (runs of at most 2^15 of their
+//                           pairs, the unit of work of the next passes)
 //   msm_partition_kernel    per 4096-pair chunk: local sort by partition,
 //                           one atomic per partition reserves the chunk's run
-//   msm_bucket_sort_kernel  per partition: bucket counts and offsets, then
-//                           the entries i | sign << 31 in bucket order; and
+//   msm_bucket_count_kernel per slice: bucket counts (LDS histogram, one
+//                           global atomic per bucket and slice)
+//   msm_bucket_offsets_kernel per partition: bucket offsets and cursors, and
 //                           each bucket's segments (runs of its entries, the
 //                           bucket sums' unit of work)
+//   msm_bucket_scatter_kernel per slice: reserves each bucket's run with one
+//                           atomic, then writes the entries i | sign << 31 in
+//                           bucket order
+// A partition that holds most of a window's pairs (skewed scalars) is so
+// spread over many workgroups; the order of a bucket's entries is that of the
+// reservations, which the sums do not depend on.
 constexpr uint32_t MSM_KEY_NONE = 0xFFFFu;
 constexpr uint32_t MSM_MAX_PARTS = 128;           // 2^(15 - 8): c <= 16
 constexpr uint32_t MSM_MAX_COUNTERS = 2048;       // windows x partitions, any c in 4..16
 constexpr uint32_t SORT_THREADS = 256;
 constexpr uint32_t SORT_PER_THREAD = 16;
 constexpr uint32_t SORT_CHUNK = SORT_THREADS * SORT_PER_THREAD;
+constexpr uint32_t SORT_SLICE_LOG = 15;  // pairs of one partition per count / scatter workgroup
 // A bucket segment is a run of entries of one bucket, the bucket sums' unit
 // of work: a bucket of more entries (skewed scalars, a sparse top window) is
 // summed by several threads and their partial sums merged. Segments are
@@ -191,35 +201,51 @@ __global__ void __launch_bounds__(256) msm_digits_kernel(const uint4* __restrict
 // Exclusive scan of the m <= MSM_MAX_COUNTERS partition sizes (window-major)
 // into part_off[0..m] (part_off[m] = the total), and the same into
 // part_cursor (the partition kernel's reservation cursors; the two may be
-// one array). One workgroup of 256. Also scans the partitions' segment
-// counts.
+// one array). With slice_off, also the exclusive scan of the partitions'
+// slice counts ceil(size / 2^SORT_SLICE_LOG) into slice_off[0..m]. One
+// workgroup of 256. Also scans the partitions' segment counts.
 __global__ void __launch_bounds__(256) msm_part_scan_kernel(const uint32_t* __restrict__ part_counts, uint32_t m,
-                                                            uint32_t* part_off, uint32_t* part_cursor) {
-  __shared__ uint32_t part[256];
+                                                            uint32_t* part_off, uint32_t* part_cursor,
+                                                            uint32_t* __restrict__ slice_off) {
+  __shared__ uint32_t part[256], spart[256];
   const uint32_t per = (m + 255) / 256;
   const uint32_t lo = threadIdx.x * per, hi = min(m, lo + per);
-  uint32_t s = 0;
-  for (uint32_t k = lo; k < hi; ++k) s += part_counts[k];
-  part[threadIdx.x] = s;
+  auto slices = [](uint32_t v) { return (v + (1u << SORT_SLICE_LOG) - 1) >> SORT_SLICE_LOG; };
+  uint32_t sum = 0, ssum = 0;
+  for (uint32_t k = lo; k < hi; ++k) {
+    const uint32_t v = part_counts[k];
+    sum += v;
+    ssum += slices(v);
+  }
+  part[threadIdx.x] = sum;
+  spart[threadIdx.x] = ssum;
   __syncthreads();
   if (threadIdx.x == 0) {
-    uint32_t run = 0;
+    uint32_t run = 0, srun = 0;
     for (uint32_t t = 0; t < 256; ++t) {
-      const uint32_t v = part[t];
+      const uint32_t v = part[t], sv = spart[t];
       part[t] = run;
+      spart[t] = srun;
       run += v;
+      srun += sv;
     }
   }
   __syncthreads();
-  uint32_t run = part[threadIdx.x];
+  uint32_t run = part[threadIdx.x], srun = spart[threadIdx.x];
   for (uint32_t k = lo; k < hi; ++k) {
+    const uint32_t v = part_counts[k];
     part_off[k] = run;
     part_cursor[k] = run;
-    run += part_counts[k];
+    run += v;
+    if (slice_off) {
+      slice_off[k] = srun;
+      srun += slices(v);
+    }
   }
   if (threadIdx.x == 255) {  // its range ends at m: run is the total
     part_off[m] = run;
     part_cursor[m] = run;
+    if (slice_off) slice_off[m] = srun;
   }
 }
 
@@ -275,29 +301,59 @@ __global__ void __launch_bounds__(SORT_THREADS) msm_partition_kernel(const uint1
   }
 }
 
-// One workgroup per partition (window w, partition p): the counts and list
-// offsets of its 2^lo_bits buckets (buckets w * half + (p << lo_bits) + lo),
-// then its entries into bucket order, a SORT_CHUNK chunk at a time sorted in
-// LDS so that each bucket's run is written by consecutive threads.
-__global__ void __launch_bounds__(SORT_THREADS) msm_bucket_sort_kernel(
-    const uint32_t* __restrict__ part_off, const uint32_t* __restrict__ part_counts,
-    const uint32_t* __restrict__ t_entry, const uint8_t* __restrict__ t_lo, uint32_t c, uint32_t lo_bits,
-    uint32_t* __restrict__ counts, uint32_t* __restrict__ offsets, uint32_t* __restrict__ list,
-    uint32_t seg_log, uint32_t* __restrict__ seg_local, uint32_t* __restrict__ part_segs,
-    uint32_t* __restrict__ seg_max) {
-  __shared__ uint32_t cur[256], lcnt[256], loff[256];
-  __shared__ uint32_t st_entry[SORT_CHUNK];
-  __shared__ uint8_t st_lo[SORT_CHUNK];
-  const uint32_t g = blockIdx.x, hi_bits = c - 1 - lo_bits;
-  const uint32_t w = g >> hi_bits, p = g & ((1u << hi_bits) - 1u);
+// The slice a count / scatter workgroup covers: partition g (the largest
+// with slice_off[g] <= the workgroup; empty partitions have no slices) and
+// the pairs [*start, *end) of the partitioned arrays. False past the last
+// slice (the grid is a bound on the slices).
+__device__ __forceinline__ bool sort_slice(const uint32_t* slice_off, const uint32_t* part_off,
+                                           const uint32_t* part_counts, uint32_t m, uint32_t wg, uint32_t* g,
+                                           uint32_t* start, uint32_t* end) {
+  if (wg >= slice_off[m]) return false;
+  uint32_t lo = 0, hi = m;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (slice_off[mid] <= wg) lo = mid; else hi = mid;
+  }
+  *g = lo;
+  const uint32_t first = part_off[lo] + ((wg - slice_off[lo]) << SORT_SLICE_LOG);
+  *start = first;
+  *end = min(part_off[lo] + part_counts[lo], first + (1u << SORT_SLICE_LOG));
+  return true;
+}
+
+// One workgroup per slice: the slice's bucket counts in LDS, added to the
+// global counts (bucket g * 2^lo_bits + low bits for partition g; zeroed by
+// the caller).
+__global__ void __launch_bounds__(SORT_THREADS) msm_bucket_count_kernel(
+    const uint32_t* __restrict__ slice_off, const uint32_t* __restrict__ part_off,
+    const uint32_t* __restrict__ part_counts, uint32_t m, const uint8_t* __restrict__ t_lo, uint32_t lo_bits,
+    uint32_t* __restrict__ counts) {
+  __shared__ uint32_t hist[256];
+  uint32_t g, start, end;
+  if (!sort_slice(slice_off, part_off, part_counts, m, blockIdx.x, &g, &start, &end)) return;
   const uint32_t nlo = 1u << lo_bits;
-  const uint32_t start = part_off[g], m = part_counts[g];
-  for (uint32_t l = threadIdx.x; l < nlo; l += SORT_THREADS) lcnt[l] = 0;
+  for (uint32_t l = threadIdx.x; l < nlo; l += SORT_THREADS) hist[l] = 0;
   __syncthreads();
-  for (uint32_t j = threadIdx.x; j < m; j += SORT_THREADS) atomicAdd(&lcnt[t_lo[start + j]], 1u);
+  for (uint32_t j = start + threadIdx.x; j < end; j += SORT_THREADS) atomicAdd(&hist[t_lo[j]], 1u);
+  __syncthreads();
+  for (uint32_t l = threadIdx.x; l < nlo; l += SORT_THREADS)
+    if (hist[l]) atomicAdd(&counts[(g << lo_bits) + l], hist[l]);
+}
+
+// One workgroup per partition: its buckets' list offsets (and the scatter's
+// cursors, the same), and their segments: each bucket's first segment within
+// the partition, the partition's segment count, and the device maximum of
+// the segments of one bucket.
+__global__ void __launch_bounds__(SORT_THREADS) msm_bucket_offsets_kernel(
+    const uint32_t* __restrict__ part_off, const uint32_t* __restrict__ counts, uint32_t lo_bits, uint32_t seg_log,
+    uint32_t* __restrict__ offsets, uint32_t* __restrict__ cursor, uint32_t* __restrict__ seg_local,
+    uint32_t* __restrict__ part_segs, uint32_t* __restrict__ seg_max) {
+  __shared__ uint32_t lcnt[256], cur[256], loff[256];
+  const uint32_t g = blockIdx.x, nlo = 1u << lo_bits, b0 = g << lo_bits;
+  for (uint32_t l = threadIdx.x; l < nlo; l += SORT_THREADS) lcnt[l] = counts[b0 + l];
   __syncthreads();
   if (threadIdx.x == 0) {
-    uint32_t run = start, segs = 0, most = 0;
+    uint32_t run = part_off[g], segs = 0, most = 0;
     for (uint32_t l = 0; l < nlo; ++l) {
       cur[l] = run;
       run += lcnt[l];
@@ -310,16 +366,36 @@ __global__ void __launch_bounds__(SORT_THREADS) msm_bucket_sort_kernel(
     if (most > 1) atomicMax(seg_max, most);
   }
   __syncthreads();
-  const uint32_t half = 1u << (c - 1);
   for (uint32_t l = threadIdx.x; l < nlo; l += SORT_THREADS) {
-    const uint32_t b = w * half + (p << lo_bits) + l;
-    counts[b] = lcnt[l];
-    offsets[b] = cur[l];
-    seg_local[b] = loff[l];
+    offsets[b0 + l] = cur[l];
+    cursor[b0 + l] = cur[l];
+    seg_local[b0 + l] = loff[l];
   }
-  for (uint32_t c0 = 0; c0 < m; c0 += SORT_CHUNK) {
-    const uint32_t len = min(SORT_CHUNK, m - c0);
-    __syncthreads();  // the previous chunk's reads of lcnt / loff / staging are done
+}
+
+// One workgroup per slice: its bucket counts again, one atomic per bucket on
+// the cursors reserves the slice's run of each bucket, then the slice goes
+// into those runs a SORT_CHUNK chunk at a time, sorted by bucket in LDS so
+// that consecutive threads write each run.
+__global__ void __launch_bounds__(SORT_THREADS) msm_bucket_scatter_kernel(
+    const uint32_t* __restrict__ slice_off, const uint32_t* __restrict__ part_off,
+    const uint32_t* __restrict__ part_counts, uint32_t m, const uint32_t* __restrict__ t_entry,
+    const uint8_t* __restrict__ t_lo, uint32_t lo_bits, uint32_t* __restrict__ cursor, uint32_t* __restrict__ list) {
+  __shared__ uint32_t cur[256], lcnt[256], loff[256];
+  __shared__ uint32_t st_entry[SORT_CHUNK];
+  __shared__ uint8_t st_lo[SORT_CHUNK];
+  uint32_t g, start, end;
+  if (!sort_slice(slice_off, part_off, part_counts, m, blockIdx.x, &g, &start, &end)) return;
+  const uint32_t nlo = 1u << lo_bits;
+  for (uint32_t l = threadIdx.x; l < nlo; l += SORT_THREADS) lcnt[l] = 0;
+  __syncthreads();
+  for (uint32_t j = start + threadIdx.x; j < end; j += SORT_THREADS) atomicAdd(&lcnt[t_lo[j]], 1u);
+  __syncthreads();
+  for (uint32_t l = threadIdx.x; l < nlo; l += SORT_THREADS)
+    cur[l] = lcnt[l] ? atomicAdd(&cursor[(g << lo_bits) + l], lcnt[l]) : 0u;
+  for (uint32_t c0 = start; c0 < end; c0 += SORT_CHUNK) {
+    const uint32_t len = min(SORT_CHUNK, end - c0);
+    __syncthreads();  // cur is set; the previous chunk's reads of lcnt / loff / staging are done
     for (uint32_t l = threadIdx.x; l < nlo; l += SORT_THREADS) lcnt[l] = 0;
     __syncthreads();
     uint32_t lo[SORT_PER_THREAD], rank[SORT_PER_THREAD], ent[SORT_PER_THREAD];
@@ -327,8 +403,8 @@ __global__ void __launch_bounds__(SORT_THREADS) msm_bucket_sort_kernel(
     for (uint32_t q = 0; q < SORT_PER_THREAD; ++q) {
       const uint32_t j = q * SORT_THREADS + threadIdx.x;
       if (j < len) {
-        lo[q] = t_lo[start + c0 + j];
-        ent[q] = t_entry[start + c0 + j];
+        lo[q] = t_lo[c0 + j];
+        ent[q] = t_entry[c0 + j];
         rank[q] = atomicAdd(&lcnt[lo[q]], 1u);
       }
     }
@@ -559,21 +635,27 @@ hipError_t launch_msm_digits(const void* scalars, const void* bases, uint32_t n,
                                          (BaseLimbs*)blimbs, keys, part_counts);
   return hipGetLastError();
 }
+// upper bound of the slices of `pairs` pairs over m partitions
+uint64_t msm_max_slices(uint64_t pairs, uint32_t m) { return (pairs >> SORT_SLICE_LOG) + m; }
 hipError_t launch_msm_sort(const uint16_t* keys, uint32_t n, uint32_t c, uint32_t windows, uint32_t lo_bits,
-                           const uint32_t* part_counts, uint32_t* part_off, uint32_t* part_cursor, uint32_t* t_entry,
-                           uint8_t* t_lo, uint32_t* counts, uint32_t* offsets, uint32_t* list, uint32_t seg_log,
-                           uint32_t* seg_local, uint32_t* part_segs, uint32_t* part_seg_off, uint32_t* seg_max,
-                           hipStream_t s) {
+                           const uint32_t* part_counts, uint32_t* part_off, uint32_t* part_cursor,
+                           uint32_t* slice_off, uint32_t* t_entry, uint8_t* t_lo, uint32_t* counts, uint32_t* offsets,
+                           uint32_t* cursor, uint32_t* list, uint32_t seg_log, uint32_t* seg_local,
+                           uint32_t* part_segs, uint32_t* part_seg_off, uint32_t* seg_max, hipStream_t s) {
   if (c < 2 || c > 16 || lo_bits > 8 || lo_bits > c - 1) return hipErrorInvalidValue;
-  const uint32_t parts = 1u << (c - 1 - lo_bits);
-  if (parts > MSM_MAX_PARTS || windows * parts > MSM_MAX_COUNTERS) return hipErrorInvalidValue;
-  msm_part_scan_kernel<<<1, 256, 0, s>>>(part_counts, windows * parts, part_off, part_cursor);
+  const uint32_t parts = 1u << (c - 1 - lo_bits), m = windows * parts;
+  if (parts > MSM_MAX_PARTS || m > MSM_MAX_COUNTERS) return hipErrorInvalidValue;
+  const uint32_t slices = (uint32_t)msm_max_slices((uint64_t)windows * n, m);
+  msm_part_scan_kernel<<<1, 256, 0, s>>>(part_counts, m, part_off, part_cursor, slice_off);
   msm_partition_kernel<<<dim3((n + SORT_CHUNK - 1) / SORT_CHUNK, windows), SORT_THREADS, 0, s>>>(
       keys, n, c, lo_bits, part_cursor, t_entry, t_lo);
-  msm_bucket_sort_kernel<<<windows * parts, SORT_THREADS, 0, s>>>(part_off, part_counts, t_entry, t_lo, c, lo_bits,
-                                                                  counts, offsets, list, seg_log, seg_local,
-                                                                  part_segs, seg_max);
-  msm_part_scan_kernel<<<1, 256, 0, s>>>(part_segs, windows * parts, part_seg_off, part_seg_off);
+  msm_bucket_count_kernel<<<slices, SORT_THREADS, 0, s>>>(slice_off, part_off, part_counts, m, t_lo, lo_bits,
+                                                          counts);
+  msm_bucket_offsets_kernel<<<m, SORT_THREADS, 0, s>>>(part_off, counts, lo_bits, seg_log, offsets, cursor,
+                                                       seg_local, part_segs, seg_max);
+  msm_part_scan_kernel<<<1, 256, 0, s>>>(part_segs, m, part_seg_off, part_seg_off, nullptr);
+  msm_bucket_scatter_kernel<<<slices, SORT_THREADS, 0, s>>>(slice_off, part_off, part_counts, m, t_entry, t_lo,
+                                                            lo_bits, cursor, list);
   return hipGetLastError();
 }
 // Segment length 2^seg_log: four times the mean bucket size (so that a
