@@ -11,9 +11,11 @@
 //   msm_digits    per point: base to the 2^261 domain, scalar to signed digits
 //                 (one 16-bit key per window)
 //   sort          the (window, point) pairs into bucket order: two radix
-//                 passes through LDS (partition scan, partition, bucket sort)
-//   msm_buckets   one thread per bucket: sum of its points (mixed additions
-//                 in XYZZ coordinates)
+//                 passes through LDS (partition scan, partition, then per
+//                 slice of a partition bucket counts and scatter)
+//   msm_buckets   one thread per bucket segment: sum of its points (mixed
+//                 additions in XYZZ coordinates); then a pairwise tree of
+//                 the segment sums of buckets that have several
 //   msm_chunks    one thread per run of L buckets of a window: running sums
 //                 give sum_j (j+1) B_j over the run
 //   msm_windows   one workgroup per window: tree sum of its runs, stored as
