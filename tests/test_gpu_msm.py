@@ -83,9 +83,11 @@ def test_vs_oracle_with_edge_bases(n):
     assert _msm_host(scalars, bases) == O.g1_msm(scalars, bases, threads=16)
 
 
-@pytest.mark.parametrize("log_n", [12, 16, 20])
-def test_closed_form_large(log_n):
-    n = 1 << log_n
+@pytest.mark.parametrize("n", [1 << 12, 1 << 16, 1 << 20, 700001, (1 << 21) + 4097])
+def test_closed_form_large(n):
+    """random scalars; the odd sizes leave partial sort chunks, slices and
+    segments"""
+    log_n = n.bit_length()
     a, d = 0x1234567 + log_n, 0x9876543210FEDCBA
     bases = O.g1_arith_bases(n, a, d)
     scalars = O.gen(4500 + log_n, n)
