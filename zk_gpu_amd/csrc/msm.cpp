@@ -246,7 +246,7 @@ MsmPlan msm_plan(uint32_t n) {
   p.off_seg_sums = take(p.max_segs * msm_point_limb_bytes());
   p.off_seg_bucket = take(p.max_segs * 4);
   p.off_buckets = take(nb * msm_point_limb_bytes());
-  p.off_parts = take((size_t)p.windows * p.runs * msm_point_limb_bytes());
+  p.off_parts = take((size_t)p.windows * p.runs * 2 * msm_point_limb_bytes());  // two parts per run
   p.off_wout = take((size_t)p.windows * 96);
   p.bytes = off;
   return p;

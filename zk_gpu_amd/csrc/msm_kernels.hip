@@ -16,7 +16,7 @@
 //   msm_buckets   one thread per bucket segment: sum of its points (mixed
 //                 additions in XYZZ coordinates); then a pairwise tree of
 //                 the segment sums of buckets that have several
-//   msm_chunks    one thread per run of L buckets of a window: running sums
+//   msm_chunks    two threads per run of L buckets of a window: running sums
 //                 give sum_j (j+1) B_j over the run
 //   msm_windows   one workgroup per window: tree sum of its runs, stored as
 //                 canonical halo2 words
@@ -522,41 +522,56 @@ __global__ void msm_bucket_gather_kernel(const PointLimbs* __restrict__ seg_sums
               counts[b] ? load_point(seg_sums, seg_prefix(part_seg_off, seg_local, b, lo_bits)) : g1_identity());
 }
 
-// One thread per run of `run` buckets [lo, lo + run) of window w:
-// sum_{j in run} (j + 1) B_j = T + lo * S, where the running sums over the run
-// (from its top down) give S = sum B_j and T = sum (j - lo + 1) B_j.
-__global__ void msm_chunks_kernel(const PointLimbs* __restrict__ buckets, uint32_t half, uint32_t run,
-                                  uint32_t runs_per_window, uint32_t total_runs, PointLimbs* __restrict__ out) {
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= total_runs) return;
-  const uint32_t w = t / runs_per_window, k = t % runs_per_window;
+// Two threads, in different waves, per run of `run` buckets [lo, lo + run)
+// of window w: sum_{j in run} (j + 1) B_j = T + lo * S, where the running
+// sums over the run (from its top down) give T = sum (j - lo + 1) B_j, and
+// S = sum B_j. One thread computes T (two additions per bucket), the other S
+// and lo * S (one addition per bucket, then double-and-add), so that neither
+// chain is longer than about two additions per bucket: the kernel has few
+// threads, and its time is the chain's latency. Outputs T and lo * S as
+// parts 2r and 2r + 1 of run r. Workgroups of 64: workgroup 2q + role takes
+// runs 64q .. 64q + 63.
+__global__ void __launch_bounds__(64) msm_chunks_kernel(const PointLimbs* __restrict__ buckets, uint32_t half,
+                                                        uint32_t run, uint32_t runs_per_window, uint32_t total_runs,
+                                                        PointLimbs* __restrict__ out) {
+  const uint32_t role = blockIdx.x & 1u, r = (blockIdx.x >> 1) * 64 + threadIdx.x;
+  if (r >= total_runs) return;
+  const uint32_t w = r / runs_per_window, k = r % runs_per_window;
   const uint32_t lo = k * run;
-  g1j S = g1_identity(), T = g1_identity();
-  for (uint32_t j = run; j-- > 0;) {
-    S = g1_add(S, load_point(buckets, w * half + lo + j));
-    T = g1_add(T, S);
+  g1j S = g1_identity();
+  if (role == 0) {
+    g1j T = g1_identity();
+    for (uint32_t j = run; j-- > 0;) {
+      S = g1_add(S, load_point(buckets, w * half + lo + j));
+      T = g1_add(T, S);
+    }
+    store_point(out, 2 * r, T);
+  } else {
+    for (uint32_t j = run; j-- > 0;) S = g1_add(S, load_point(buckets, w * half + lo + j));
+    // lo * S by double-and-add, most significant bit first
+    g1j L = g1_identity();
+    for (int bit = lo ? 31 - __clz(lo) : -1; bit >= 0; --bit) {
+      L = g1_double(L);
+      if ((lo >> bit) & 1u) L = g1_add(L, S);
+    }
+    store_point(out, 2 * r + 1, L);
   }
-  // lo * S by double-and-add, most significant bit first
-  g1j L = g1_identity();
-  for (int bit = lo ? 31 - __clz(lo) : -1; bit >= 0; --bit) {
-    L = g1_double(L);
-    if ((lo >> bit) & 1u) L = g1_add(L, S);
-  }
-  store_point(out, t, g1_add(T, L));
 }
 
-// One workgroup (256 threads) per window: tree sum of its runs' partial sums;
-// the window's sum goes out as canonical halo2 words X, Y, Z (96 bytes).
-__global__ void __launch_bounds__(256) msm_windows_kernel(const PointLimbs* __restrict__ parts,
-                                                          uint32_t runs_per_window, uint4* __restrict__ out) {
-  __shared__ PointLimbs tree[256];
+// One workgroup (512 threads) per window: tree sum of its parts (two per
+// run); the window's sum goes out as canonical halo2 words X, Y, Z (96 bytes).
+constexpr uint32_t WINDOW_THREADS = 512;
+__global__ void __launch_bounds__(WINDOW_THREADS) msm_windows_kernel(const PointLimbs* __restrict__ parts,
+                                                                     uint32_t parts_per_window,
+                                                                     uint4* __restrict__ out) {
+  __shared__ PointLimbs tree[WINDOW_THREADS];
   const uint32_t w = blockIdx.x;
   g1j acc = g1_identity();
-  for (uint32_t k = threadIdx.x; k < runs_per_window; k += 256)
-    acc = g1_add(acc, load_point(parts, w * runs_per_window + k));
+  for (uint32_t k = threadIdx.x; k < parts_per_window; k += WINDOW_THREADS)
+    acc = g1_add(acc, load_point(parts, w * parts_per_window + k));
   store_point(tree, threadIdx.x, acc);
   __syncthreads();
-  for (uint32_t off = 128; off > 0; off >>= 1) {
+  for (uint32_t off = WINDOW_THREADS / 2; off > 0; off >>= 1) {
     if (threadIdx.x < off) {
       const g1j a = g1_add(load_point(tree, threadIdx.x), load_point(tree, threadIdx.x + off));
       store_point(tree, threadIdx.x, a);
@@ -696,16 +711,18 @@ hipError_t launch_msm_bucket_merge(void* seg_sums, const uint32_t* seg_bucket, c
                                                                seg_local, lo_bits, nbuckets, (PointLimbs*)buckets);
   return hipGetLastError();
 }
+// parts: 2 x runs_per_window x windows point entries
 hipError_t launch_msm_chunks(const void* buckets, uint32_t half, uint32_t run, uint32_t runs_per_window,
                              uint32_t windows, void* parts, hipStream_t s) {
   const uint32_t total = runs_per_window * windows;
-  msm_chunks_kernel<<<(total + 63) / 64, 64, 0, s>>>((const PointLimbs*)buckets, half, run, runs_per_window, total,
-                                                    (PointLimbs*)parts);
+  msm_chunks_kernel<<<2 * ((total + 63) / 64), 64, 0, s>>>((const PointLimbs*)buckets, half, run, runs_per_window,
+                                                          total, (PointLimbs*)parts);
   return hipGetLastError();
 }
 hipError_t launch_msm_windows(const void* parts, uint32_t runs_per_window, uint32_t windows, void* out,
                               hipStream_t s) {
-  msm_windows_kernel<<<windows, 256, 0, s>>>((const PointLimbs*)parts, runs_per_window, (uint4*)out);
+  msm_windows_kernel<<<windows, WINDOW_THREADS, 0, s>>>((const PointLimbs*)parts, 2 * runs_per_window,
+                                                        (uint4*)out);
   return hipGetLastError();
 }
 size_t msm_base_limb_bytes() { return sizeof(BaseLimbs); }
