@@ -56,8 +56,7 @@ __device__ __forceinline__ void fe_to3(const fe& v, uint4& a, uint4& b, uint4& c
   c = make_uint4(v.l[8], 0u, 0u, 0u);
 }
 // the base's coordinates as normalized limbs, value below q
-__device__ __forceinline__ g1a load_base(const BaseLimbs* b, uint32_t i) {
-  const BaseLimbs e = b[i];
+__device__ __forceinline__ g1a unpack_base(const BaseLimbs& e) {
   const uint32_t xw[8] = {e.x0.x, e.x0.y, e.x0.z, e.x0.w, e.x1.x, e.x1.y, e.x1.z, e.x1.w};
   const uint32_t yw[8] = {e.y0.x, e.y0.y, e.y0.z, e.y0.w, e.y1.x, e.y1.y, e.y1.z, e.y1.w & 0x7fffffffu};
   g1a p;
@@ -66,6 +65,7 @@ __device__ __forceinline__ g1a load_base(const BaseLimbs* b, uint32_t i) {
   p.inf = (e.y1.w >> 31) != 0;
   return p;
 }
+__device__ __forceinline__ g1a load_base(const BaseLimbs* b, uint32_t i) { return unpack_base(b[i]); }
 __device__ __forceinline__ g1j load_point(const PointLimbs* p, uint32_t i) {
   const PointLimbs e = p[i];
   return g1j{fe_from3(e.a[0], e.a[1], e.a[2]), fe_from3(e.a[3], e.a[4], e.a[5]), fe_from3(e.a[6], e.a[7], e.a[8])};
@@ -449,7 +449,7 @@ __device__ __forceinline__ uint32_t seg_prefix(const uint32_t* part_seg_off, con
 // sum of its points, accumulated in XYZZ coordinates, stored in Jacobian ones.
 // The segment's bucket is the largest b whose first segment is <= s (empty
 // buckets have no segments, so that bucket is never empty).
-__global__ void msm_buckets_kernel(const BaseLimbs* __restrict__ blimbs, const uint32_t* __restrict__ offsets,
+__global__ void __launch_bounds__(64) msm_buckets_kernel(const BaseLimbs* __restrict__ blimbs, const uint32_t* __restrict__ offsets,
                                    const uint32_t* __restrict__ counts, const uint32_t* __restrict__ list,
                                    const uint32_t* __restrict__ part_seg_off, const uint32_t* __restrict__ seg_local,
                                    uint32_t nparts, uint32_t lo_bits, uint32_t seg_log, uint32_t nbuckets,
@@ -468,19 +468,22 @@ __global__ void msm_buckets_kernel(const BaseLimbs* __restrict__ blimbs, const u
   const uint32_t start = offsets[b] + first, cnt = min(1u << lg, count - first);
   g1xyzz acc;
   bool inf = true;
-  // software pipeline: the list entry two ahead and the base one ahead are
-  // in flight while the current addition runs
-  uint32_t e = cnt > 0 ? list[start] : 0u;
-  uint32_t e_next = cnt > 1 ? list[start + 1] : 0u;
-  g1a base = load_base(blimbs, cnt > 0 ? e & 0x7fffffffu : 0u);
+  // software pipeline: the list entry two ahead and the base record one
+  // ahead are in flight while the current addition runs. The record stays
+  // packed until its iteration, and the loads are unconditional (indices
+  // clamped to the segment's last entry): unpacking a record where it is
+  // loaded, or a load under a branch, would make the wave wait for the load
+  // right there. A segment has at least one entry.
+  const uint32_t last = start + cnt - 1;
+  uint32_t e = list[start];
+  uint32_t e_next = list[min(start + 1, last)];
+  BaseLimbs raw = blimbs[e & 0x7fffffffu];
   for (uint32_t k = 0; k < cnt; ++k) {
-    const g1a cur = base;
+    const g1a cur = unpack_base(raw);
     const bool neg = (e >> 31) != 0;
-    if (k + 1 < cnt) {
-      base = load_base(blimbs, e_next & 0x7fffffffu);
-      e = e_next;
-      e_next = k + 2 < cnt ? list[start + k + 2] : 0u;
-    }
+    raw = blimbs[e_next & 0x7fffffffu];
+    e = e_next;
+    e_next = list[min(start + k + 2, last)];
     g1_bucket_add(acc, inf, cur, neg);
   }
   store_point(seg_sums, sidx, inf ? g1_identity() : g1_xyzz_to_jacobian(acc));
