@@ -146,6 +146,56 @@ def _mix64(z):
     return z ^ (z >> 31)
 
 
+def _splitmix_sum(n, seed, start=0):
+    """sum of the synthetic base multipliers k_i (include/zkgpu_ntt.h:
+    SplitMix64 of (seed, start + i), low bit set), as an integer"""
+    total = 0
+    step = 1 << 24
+    with np.errstate(over="ignore"):
+        for lo in range(0, n, step):
+            i = np.arange(start + lo, start + min(n, lo + step), dtype=np.uint64)
+            z = np.uint64(seed) * np.uint64(0xD1B54A32D192ED03) + i * np.uint64(0x9E3779B97F4A7C15)
+            z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+            z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+            k = (z ^ (z >> np.uint64(31))) | np.uint64(1)
+            total += int((k >> np.uint64(32)).sum()) << 32
+            total += int((k & np.uint64(0xFFFFFFFF)).sum())
+    return total
+
+
+@pytest.mark.parametrize("n", [(1 << 27) + 3, 1 << 28])
+def test_pieces_at_the_largest_sizes(torch_cuda, n):
+    """Above 2^27 points the MSM runs as pieces (two here, the first of
+    2^26 + 2 points at 2^27 + 3) whose sums the host adds; 2^28 is the largest
+    length. Equal scalars v over synthetic bases k_i G: the sum is
+    (v sum_i k_i mod r) G."""
+    torch = torch_cuda
+    seed, v = 77, 0x2468ACE013579BDF2468ACE013579BDF
+    db = torch.empty(n * 8, dtype=torch.int64, device="cuda")
+    Z.fill_random_points(ptr(db), n, seed, 0)
+    w = torch.from_numpy(O.elem(v).view(np.int64).reshape(1, 4).copy()).to("cuda")
+    ds = w.expand(n, 4).contiguous()
+    torch.cuda.synchronize()
+    acc = Z.msm_device(ptr(ds), ptr(db), n)
+    del ds, db
+    torch.cuda.empty_cache()
+    assert _affine(acc) == O.g1_mul_gen(v * _splitmix_sum(n, seed) % R)
+
+
+def test_host_api_pieces(torch_cuda):
+    """The host-buffer entry point stages and sums pieces of at most 2^24
+    points: 2^24 + 5 points run as two."""
+    torch = torch_cuda
+    n, seed, v = (1 << 24) + 5, 91, 0x13579BDF2468ACE0
+    db = torch.empty(n * 8, dtype=torch.int64, device="cuda")
+    Z.fill_random_points(ptr(db), n, seed, 0)
+    torch.cuda.synchronize()
+    bases = np.ascontiguousarray(db.cpu().numpy().view(np.uint64).reshape(n, 8))
+    del db
+    scalars = np.ascontiguousarray(np.tile(O.elem(v), (n, 1)))
+    assert _msm_host(scalars, bases) == O.g1_mul_gen(v * _splitmix_sum(n, seed) % R)
+
+
 def test_fill_random_points(torch_cuda):
     """Synthetic bases: point i = k_i G with k_i from SplitMix64 (the rule in
     include/zkgpu_ntt.h), checked against the oracle's k G."""

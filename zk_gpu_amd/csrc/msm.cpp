@@ -314,9 +314,8 @@ int msm_enqueue(Engine* eng, const MsmPlan& p, const void* d_scalars, const void
   return ZK_OK;
 }
 
-// Horner over the window sums (halo2 words, 12 u64 each); out: X, Y, Z with
-// Z = 1, or all zero for the identity.
-void msm_combine(const MsmPlan& p, const uint64_t* wsum, uint64_t out[12]) {
+// Horner over the window sums (halo2 words, 12 u64 each), added to acc.
+void msm_combine(const MsmPlan& p, const uint64_t* wsum, Jac& total) {
   Jac acc = j_identity();
   for (uint32_t w = p.windows; w-- > 0;) {
     for (uint32_t k = 0; k < p.c; ++k) acc = j_double(acc);
@@ -326,6 +325,11 @@ void msm_combine(const MsmPlan& p, const uint64_t* wsum, uint64_t out[12]) {
     memcpy(s.z.w, wsum + 12 * w + 8, 32);
     acc = j_add(acc, s);
   }
+  total = j_add(total, acc);
+}
+
+// out: X, Y, Z with Z = 1, or all zero for the identity.
+void msm_affine(const Jac& acc, uint64_t out[12]) {
   memset(out, 0, 96);
   if (j_is_identity(acc)) return;
   const Q zi = q_inv(acc.z), zi2 = q_mul(zi, zi), zi3 = q_mul(zi2, zi);
@@ -333,6 +337,48 @@ void msm_combine(const MsmPlan& p, const uint64_t* wsum, uint64_t out[12]) {
   memcpy(out, x.w, 32);
   memcpy(out + 4, y.w, 32);
   memcpy(out + 8, QONE.w, 32);
+}
+
+// An MSM of more than 2^27 points runs as pieces of at most that many, one
+// device pass each, their sums added on the host: a pass then has at most
+// 16 x 2^27 = 2^31 (window, point) pairs, so every 32-bit position, total and
+// loop bound of the sort stays far from wrapping.
+constexpr uint64_t MSM_MAX_PIECE = 1ull << 27;
+// The host-buffer entry point stages and sums pieces of at most 2^24 points,
+// so that its staging buffer and workspace (about 6 GB) stay within the
+// host-call HBM budget at any length.
+constexpr uint64_t MSM_HOST_PIECE = 1ull << 24;
+struct MsmPieces {
+  uint64_t count = 0, size = 0;  // pieces of `size` points, the last one shorter
+  size_t bytes = 0;              // workspace for the largest of them
+};
+MsmPieces msm_pieces(uint64_t length, uint64_t max_piece = MSM_MAX_PIECE) {
+  MsmPieces q;
+  q.count = (length + max_piece - 1) / max_piece;
+  q.size = (length + q.count - 1) / q.count;
+  const uint64_t last = length - (q.count - 1) * q.size;
+  q.bytes = std::max(msm_plan((uint32_t)q.size).bytes, msm_plan((uint32_t)last).bytes);
+  return q;
+}
+
+// The pieces of an MSM on device operands: each enqueued on s, its window
+// sums downloaded and combined into total.
+int msm_run(Engine* e, const char* d_scalars, const char* d_bases, uint64_t length, void* ws, hipStream_t s,
+            Jac& total) {
+  const MsmPieces q = msm_pieces(length);
+  total = j_identity();
+  for (uint64_t off = 0; off < length; off += q.size) {
+    const MsmPlan p = msm_plan((uint32_t)std::min(q.size, length - off));
+    int st = msm_enqueue(e, p, d_scalars + off * 32, d_bases + off * 64, ws, s);
+    std::vector<uint64_t> wsum((size_t)p.windows * 12);
+    if (st == ZK_OK &&
+        hipMemcpyAsync(wsum.data(), (char*)ws + p.off_wout, wsum.size() * 8, hipMemcpyDeviceToHost, s) != hipSuccess)
+      st = ZK_ERR_HIP;
+    if (hipStreamSynchronize(s) != hipSuccess && st == ZK_OK) st = ZK_ERR_HIP;
+    if (st) return st;
+    msm_combine(p, wsum.data(), total);
+  }
+  return ZK_OK;
 }
 
 }  // namespace zkgpu
@@ -351,18 +397,12 @@ int zkgpu_msm_device(int device_id, const void* d_scalars, const void* d_bases, 
   Engine* e = Engine::get(device_id, &st);
   if (!e) return st;
   hipStream_t s = (hipStream_t)stream;
-  const MsmPlan p = msm_plan((uint32_t)length);
-  WsLease lease = e->lease_workspace(s, p.bytes, WS_MSM, &st);
+  WsLease lease = e->lease_workspace(s, msm_pieces(length).bytes, WS_MSM, &st);
   if (st) return st;
-  st = msm_enqueue(e, p, d_scalars, d_bases, lease.ptr(), s);
-  std::vector<uint64_t> wsum((size_t)p.windows * 12);
-  if (st == ZK_OK &&
-      hipMemcpyAsync(wsum.data(), (char*)lease.ptr() + p.off_wout, wsum.size() * 8, hipMemcpyDeviceToHost, s) !=
-          hipSuccess)
-    st = ZK_ERR_HIP;
-  if (hipStreamSynchronize(s) != hipSuccess && st == ZK_OK) st = ZK_ERR_HIP;
+  Jac total;
+  st = msm_run(e, (const char*)d_scalars, (const char*)d_bases, length, lease.ptr(), s, total);
   if (st) return st;
-  msm_combine(p, wsum.data(), acc);
+  msm_affine(total, acc);
   return ZK_OK;
 }
 
@@ -376,23 +416,25 @@ uint64_t parallel_msm_api(size_t device_id, const uint64_t* data, const void* ba
   int st = ZK_OK;
   Engine* e = Engine::get((int)device_id, &st);
   if (!e) return (uint64_t)st;
-  const size_t sbytes = (size_t)length * 32, bbytes = (size_t)length * 64;
-  const MsmPlan p = msm_plan((uint32_t)length);
-  HostLease ctx(e, sbytes + bbytes, 0, p.bytes);
+  const MsmPieces q = msm_pieces(length, MSM_HOST_PIECE);
+  HostLease ctx(e, q.size * 96, 0, q.bytes);
   if (ctx.status()) return (uint64_t)ctx.status();
   hipStream_t s = ctx->stream;
   char* io = (char*)ctx->io->ptr;
-  st = stage_h2d((int)device_id, s, io, data, sbytes);
-  if (st == ZK_OK) st = stage_h2d((int)device_id, s, io + sbytes, bases, bbytes);
-  if (st == ZK_OK) st = msm_enqueue(e, p, io, io + sbytes, ctx->aux->ptr, s);
-  std::vector<uint64_t> wsum((size_t)p.windows * 12);
-  if (st == ZK_OK &&
-      hipMemcpyAsync(wsum.data(), (char*)ctx->aux->ptr + p.off_wout, wsum.size() * 8, hipMemcpyDeviceToHost, s) !=
-          hipSuccess)
-    st = ZK_ERR_HIP;
-  if (hipStreamSynchronize(s) != hipSuccess && st == ZK_OK) st = ZK_ERR_HIP;
-  if (st) return (uint64_t)st;
-  msm_combine(p, wsum.data(), (uint64_t*)acc);
+  Jac total = j_identity();
+  for (uint64_t off = 0; off < length; off += q.size) {
+    const uint64_t m = std::min(q.size, length - off);
+    st = stage_h2d((int)device_id, s, io, data + off * 4, m * 32);
+    if (st == ZK_OK) st = stage_h2d((int)device_id, s, io + q.size * 32, (const char*)bases + off * 64, m * 64);
+    Jac part;
+    if (st == ZK_OK) st = msm_run(e, io, io + q.size * 32, m, ctx->aux->ptr, s, part);
+    if (st) {
+      (void)hipStreamSynchronize(s);
+      return (uint64_t)st;
+    }
+    total = j_add(total, part);
+  }
+  msm_affine(total, (uint64_t*)acc);
   return ZK_OK;
 }
 
@@ -407,9 +449,9 @@ int zkgpu_fill_random_points(int device_id, void* d_out, uint64_t count, uint64_
 
 uint64_t zkgpu_msm_workspace_bytes(uint64_t length, uint32_t* window_bits) {
   if (length == 0 || length > (1ull << 28)) return 0;
-  const MsmPlan p = msm_plan((uint32_t)length);
-  if (window_bits) *window_bits = p.c;
-  return p.bytes;
+  const MsmPieces q = msm_pieces(length);
+  if (window_bits) *window_bits = msm_plan((uint32_t)q.size).c;
+  return q.bytes;
 }
 
 }  // extern "C"
