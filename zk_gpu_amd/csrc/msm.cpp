@@ -23,18 +23,18 @@ hipError_t launch_msm_digits(const void* scalars, const void* bases, uint32_t n,
 hipError_t launch_msm_sort(const uint16_t* keys, uint32_t n, uint32_t c, uint32_t windows, uint32_t lo_bits,
                            const uint32_t* part_counts, uint32_t* part_off, uint32_t* part_cursor,
                            uint32_t* slice_off, uint32_t* t_entry, uint8_t* t_lo, uint32_t* counts, uint32_t* offsets,
-                           uint32_t* cursor, uint32_t* list, uint32_t seg_log, uint32_t* seg_local,
+                           uint32_t* cursor, uint32_t* list, uint32_t seg_code, uint32_t* seg_local,
                            uint32_t* part_segs, uint32_t* part_seg_off, uint32_t* seg_max, hipStream_t s);
 uint32_t msm_lo_bits(uint32_t c);
-uint32_t msm_seg_log(uint64_t pairs, uint64_t nbuckets);
-uint64_t msm_max_segments(uint64_t pairs, uint64_t nbuckets);
+uint32_t msm_seg_code(uint64_t pairs, uint64_t nbuckets);
+uint64_t msm_max_segments(uint64_t pairs, uint64_t nbuckets, uint32_t seg_code);
 hipError_t launch_msm_buckets(const void* blimbs, const uint32_t* offsets, const uint32_t* counts,
                               const uint32_t* list, const uint32_t* part_seg_off, const uint32_t* seg_local,
-                              uint32_t nparts, uint32_t lo_bits, uint32_t seg_log, uint32_t nbuckets,
+                              uint32_t nparts, uint32_t lo_bits, uint32_t seg_code, uint32_t nbuckets,
                               uint64_t max_segs, void* seg_sums, uint32_t* seg_bucket, hipStream_t s);
 hipError_t launch_msm_bucket_merge(void* seg_sums, const uint32_t* seg_bucket, const uint32_t* counts,
                                    const uint32_t* part_seg_off, const uint32_t* seg_local, const uint32_t* seg_max,
-                                   uint32_t nparts, uint32_t lo_bits, uint32_t seg_log, uint32_t n, uint32_t nbuckets,
+                                   uint32_t nparts, uint32_t lo_bits, uint32_t seg_code, uint32_t n, uint32_t nbuckets,
                                    uint64_t max_segs, void* buckets, hipStream_t s);
 hipError_t launch_msm_chunks(const void* buckets, uint32_t half, uint32_t run, uint32_t runs_per_window,
                              uint32_t windows, void* parts, hipStream_t s);
@@ -189,7 +189,7 @@ Jac j_add(const Jac& p, const Jac& q) {  // add-2007-bl
 // Window size and workspace of an n-point MSM.
 struct MsmPlan {
   uint32_t n = 0, c = 0, windows = 0, half = 0, run = 0, runs = 0, lo_bits = 0, sort_parts = 0;
-  uint32_t seg_log = 0;   // bucket segments: runs of 2^seg_log entries of one bucket (heavy ones: shorter)
+  uint32_t seg_code = 0;  // bucket segment lengths (ordinary and heavy buckets), packed for the kernels
   uint64_t max_segs = 0;  // bound on their number
   size_t off_blimbs = 0, off_keys = 0, off_tentry = 0, off_tlo = 0, off_list = 0, off_counts = 0, off_offsets = 0,
          off_pcounts = 0, off_poff = 0, off_pcursor = 0, off_slice_off = 0, off_cursor = 0, off_seg_local = 0,
@@ -204,9 +204,14 @@ MsmPlan msm_plan(uint32_t n) {
   p.n = n;
   uint32_t lg = 0;
   while ((1ull << (lg + 1)) <= n) ++lg;
-  // 2^(c-1) = n / 64 buckets per window (64 points per bucket and window on
-  // average) up to c = 16; the bucket sums dominate
-  p.c = std::max(4u, std::min(16u, lg > 5 ? lg - 5 : 4u));
+  // window width by floor(log2 n), measured on gfx950 (every c from
+  // log2 n - 8 to log2 n - 2 at 2^12..2^22 random points): below 2^21 points
+  // the latency-bound running sums and window sums favour wide windows, and
+  // widths whose top window holds only one or two bits of the 254-bit
+  // scalars (c = 11, 13, 14) lose; log2 n - 3 below 2^12
+  static const uint8_t kWidth[23] = {4, 4, 4, 4, 4, 4, 4, 4, 5, 6, 7, 8,                 // 2^0 .. 2^11
+                                     9, 9, 10, 12, 12, 15, 15, 15, 15, 16, 16};  // 2^12 .. 2^22
+  p.c = lg < 23 ? kWidth[lg] : 16u;
   p.windows = (255 + p.c - 1) / p.c;  // signed digits: W * c >= 255 bits cover every scalar below 2^254
   p.half = 1u << (p.c - 1);
   p.run = std::min(16u, p.half);
@@ -241,8 +246,8 @@ MsmPlan msm_plan(uint32_t n) {
   p.off_seg_local = take(nb * 4);
   p.off_psegs = take(m * 4);
   p.off_pseg_off = take((m + 1) * 4);
-  p.seg_log = msm_seg_log(pairs, nb);
-  p.max_segs = msm_max_segments(pairs, nb);
+  p.seg_code = msm_seg_code(pairs, nb);
+  p.max_segs = msm_max_segments(pairs, nb, p.seg_code);
   p.off_seg_sums = take(p.max_segs * msm_point_limb_bytes());
   p.off_seg_bucket = take(p.max_segs * 4);
   p.off_buckets = take(nb * msm_point_limb_bytes());
@@ -291,17 +296,17 @@ int msm_enqueue(Engine* eng, const MsmPlan& p, const void* d_scalars, const void
                            (uint32_t*)(b + p.off_poff), (uint32_t*)(b + p.off_pcursor),
                            (uint32_t*)(b + p.off_slice_off), (uint32_t*)(b + p.off_tentry), (uint8_t*)(b + p.off_tlo),
                            counts, (uint32_t*)(b + p.off_offsets), (uint32_t*)(b + p.off_cursor),
-                           (uint32_t*)(b + p.off_list), p.seg_log, seg_local, (uint32_t*)(b + p.off_psegs), pseg_off,
+                           (uint32_t*)(b + p.off_list), p.seg_code, seg_local, (uint32_t*)(b + p.off_psegs), pseg_off,
                            seg_max, s);
   });
   step("msm_buckets_kernel", [&] {
     return launch_msm_buckets(b + p.off_blimbs, (const uint32_t*)(b + p.off_offsets), counts,
-                              (const uint32_t*)(b + p.off_list), pseg_off, seg_local, m, p.lo_bits, p.seg_log, nb,
+                              (const uint32_t*)(b + p.off_list), pseg_off, seg_local, m, p.lo_bits, p.seg_code, nb,
                               p.max_segs, b + p.off_seg_sums, seg_bucket, s);
   });
   step("msm_bucket_merge (tree levels, gather)", [&] {
     return launch_msm_bucket_merge(b + p.off_seg_sums, seg_bucket, counts, pseg_off, seg_local, seg_max, m,
-                                   p.lo_bits, p.seg_log, p.n, nb, p.max_segs, b + p.off_buckets, s);
+                                   p.lo_bits, p.seg_code, p.n, nb, p.max_segs, b + p.off_buckets, s);
   });
   step("msm_chunks_kernel",
        [&] { return launch_msm_chunks(b + p.off_buckets, p.half, p.run, p.runs, p.windows, b + p.off_parts, s); });

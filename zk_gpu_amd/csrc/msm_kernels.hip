@@ -129,17 +129,20 @@ constexpr uint32_t SORT_CHUNK = SORT_THREADS * SORT_PER_THREAD;
 constexpr uint32_t SORT_SLICE_LOG = 15;  // pairs of one partition per count / scatter workgroup
 // A bucket segment is a run of entries of one bucket, the bucket sums' unit
 // of work: a bucket of more entries (skewed scalars, a sparse top window) is
-// summed by several threads and their partial sums merged. Segments are
-// 2^seg_log entries long (chosen by msm_seg_log so that ordinary buckets are
-// one segment); a heavy bucket, of more than two such segments, is cut into
-// segments of 2^MSM_HEAVY_SEG_LOG entries instead, so that a few heavy buckets
-// still give enough threads to fill the device.
-constexpr uint32_t MSM_HEAVY_SEG_LOG = 8;  // <= every seg_log (msm_seg_log's floor)
-__device__ __forceinline__ uint32_t bucket_seg_log(uint32_t count, uint32_t seg_log) {
-  return count > (2u << seg_log) ? MSM_HEAVY_SEG_LOG : seg_log;
+// summed by several threads and their partial sums merged. A plan's segment
+// lengths come packed in one word, seg_code (msm_seg_code): bits 0-7 the log2
+// of an ordinary segment, four times the mean bucket size so that a bucket of
+// uniformly random digits is one segment; bits 8-15 the log2 of a heavy
+// bucket's segments (a heavy bucket has more than two ordinary segments):
+// short where few points can make a bucket heavy, so that its threads' chains
+// of additions stay short, and longer for the largest MSMs, whose heavy
+// buckets (all-equal scalars) would otherwise make millions of segments.
+__device__ __forceinline__ uint32_t bucket_seg_log(uint32_t count, uint32_t seg_code) {
+  const uint32_t lg = seg_code & 0xFFu;
+  return count > (2u << lg) ? (seg_code >> 8) & 0xFFu : lg;
 }
-__device__ __forceinline__ uint32_t bucket_segs(uint32_t count, uint32_t seg_log) {
-  const uint32_t lg = bucket_seg_log(count, seg_log);
+__device__ __forceinline__ uint32_t bucket_segs(uint32_t count, uint32_t seg_code) {
+  const uint32_t lg = bucket_seg_log(count, seg_code);
   return (count + (1u << lg) - 1) >> lg;
 }
 
@@ -347,7 +350,7 @@ __global__ void __launch_bounds__(SORT_THREADS) msm_bucket_count_kernel(
 // the partition, the partition's segment count, and the device maximum of
 // the segments of one bucket.
 __global__ void __launch_bounds__(SORT_THREADS) msm_bucket_offsets_kernel(
-    const uint32_t* __restrict__ part_off, const uint32_t* __restrict__ counts, uint32_t lo_bits, uint32_t seg_log,
+    const uint32_t* __restrict__ part_off, const uint32_t* __restrict__ counts, uint32_t lo_bits, uint32_t seg_code,
     uint32_t* __restrict__ offsets, uint32_t* __restrict__ cursor, uint32_t* __restrict__ seg_local,
     uint32_t* __restrict__ part_segs, uint32_t* __restrict__ seg_max) {
   __shared__ uint32_t lcnt[256], cur[256], loff[256];
@@ -360,7 +363,7 @@ __global__ void __launch_bounds__(SORT_THREADS) msm_bucket_offsets_kernel(
       cur[l] = run;
       run += lcnt[l];
       loff[l] = segs;  // the bucket's first segment within the partition
-      const uint32_t bs = bucket_segs(lcnt[l], seg_log);
+      const uint32_t bs = bucket_segs(lcnt[l], seg_code);
       segs += bs;
       most = max(most, bs);
     }
@@ -452,7 +455,7 @@ __device__ __forceinline__ uint32_t seg_prefix(const uint32_t* part_seg_off, con
 __global__ void __launch_bounds__(64) msm_buckets_kernel(const BaseLimbs* __restrict__ blimbs, const uint32_t* __restrict__ offsets,
                                    const uint32_t* __restrict__ counts, const uint32_t* __restrict__ list,
                                    const uint32_t* __restrict__ part_seg_off, const uint32_t* __restrict__ seg_local,
-                                   uint32_t nparts, uint32_t lo_bits, uint32_t seg_log, uint32_t nbuckets,
+                                   uint32_t nparts, uint32_t lo_bits, uint32_t seg_code, uint32_t nbuckets,
                                    PointLimbs* __restrict__ seg_sums, uint32_t* __restrict__ seg_bucket) {
   const uint32_t sidx = blockIdx.x * blockDim.x + threadIdx.x;
   if (sidx >= part_seg_off[nparts]) return;
@@ -463,7 +466,7 @@ __global__ void __launch_bounds__(64) msm_buckets_kernel(const BaseLimbs* __rest
   }
   const uint32_t b = lo;
   seg_bucket[sidx] = b;
-  const uint32_t count = counts[b], lg = bucket_seg_log(count, seg_log);
+  const uint32_t count = counts[b], lg = bucket_seg_log(count, seg_code);
   const uint32_t first = (sidx - seg_prefix(part_seg_off, seg_local, b, lo_bits)) << lg;
   const uint32_t start = offsets[b] + first, cnt = min(1u << lg, count - first);
   g1xyzz acc;
@@ -500,7 +503,7 @@ __global__ void __launch_bounds__(64) msm_buckets_kernel(const BaseLimbs* __rest
 __global__ void __launch_bounds__(256) msm_segment_tree_kernel(
     PointLimbs* __restrict__ seg_sums, const uint32_t* __restrict__ seg_bucket, const uint32_t* __restrict__ counts,
     const uint32_t* __restrict__ part_seg_off, const uint32_t* __restrict__ seg_local,
-    const uint32_t* __restrict__ seg_max, uint32_t nparts, uint32_t lo_bits, uint32_t seg_log, uint32_t level) {
+    const uint32_t* __restrict__ seg_max, uint32_t nparts, uint32_t lo_bits, uint32_t seg_code, uint32_t level) {
   const uint32_t step = 1u << level;
   if (*seg_max <= step) return;
   const uint32_t total = part_seg_off[nparts];
@@ -508,7 +511,7 @@ __global__ void __launch_bounds__(256) msm_segment_tree_kernel(
     const uint32_t b = seg_bucket[sidx];
     const uint32_t i = sidx - seg_prefix(part_seg_off, seg_local, b, lo_bits);
     if ((i & (2 * step - 1)) != 0) continue;
-    if (i + step >= bucket_segs(counts[b], seg_log)) continue;
+    if (i + step >= bucket_segs(counts[b], seg_code)) continue;
     store_point(seg_sums, sidx, g1_add(load_point(seg_sums, sidx), load_point(seg_sums, sidx + step)));
   }
 }
@@ -660,7 +663,7 @@ uint64_t msm_max_slices(uint64_t pairs, uint32_t m) { return (pairs >> SORT_SLIC
 hipError_t launch_msm_sort(const uint16_t* keys, uint32_t n, uint32_t c, uint32_t windows, uint32_t lo_bits,
                            const uint32_t* part_counts, uint32_t* part_off, uint32_t* part_cursor,
                            uint32_t* slice_off, uint32_t* t_entry, uint8_t* t_lo, uint32_t* counts, uint32_t* offsets,
-                           uint32_t* cursor, uint32_t* list, uint32_t seg_log, uint32_t* seg_local,
+                           uint32_t* cursor, uint32_t* list, uint32_t seg_code, uint32_t* seg_local,
                            uint32_t* part_segs, uint32_t* part_seg_off, uint32_t* seg_max, hipStream_t s) {
   if (c < 2 || c > 16 || lo_bits > 8 || lo_bits > c - 1) return hipErrorInvalidValue;
   const uint32_t parts = 1u << (c - 1 - lo_bits), m = windows * parts;
@@ -671,30 +674,35 @@ hipError_t launch_msm_sort(const uint16_t* keys, uint32_t n, uint32_t c, uint32_
       keys, n, c, lo_bits, part_cursor, t_entry, t_lo);
   msm_bucket_count_kernel<<<slices, SORT_THREADS, 0, s>>>(slice_off, part_off, part_counts, m, t_lo, lo_bits,
                                                           counts);
-  msm_bucket_offsets_kernel<<<m, SORT_THREADS, 0, s>>>(part_off, counts, lo_bits, seg_log, offsets, cursor,
+  msm_bucket_offsets_kernel<<<m, SORT_THREADS, 0, s>>>(part_off, counts, lo_bits, seg_code, offsets, cursor,
                                                        seg_local, part_segs, seg_max);
   msm_part_scan_kernel<<<1, 256, 0, s>>>(part_segs, m, part_seg_off, part_seg_off, nullptr);
   msm_bucket_scatter_kernel<<<slices, SORT_THREADS, 0, s>>>(slice_off, part_off, part_counts, m, t_entry, t_lo,
                                                             lo_bits, cursor, list);
   return hipGetLastError();
 }
-// Segment length 2^seg_log: four times the mean bucket size (so that a
-// bucket of uniformly random digits is one segment), within 2^8..2^12.
-uint32_t msm_seg_log(uint64_t pairs, uint64_t nbuckets) {
+// Segment lengths (bucket_seg_log): ordinary 2^8..2^12 entries, four times
+// the mean bucket size; heavy 2^6 entries up to 2^26 (window, point) pairs,
+// then longer, up to 2^8 from 2^28 pairs.
+uint32_t msm_seg_code(uint64_t pairs, uint64_t nbuckets) {
   uint32_t lg = 8;
   while (lg < 12 && (1ull << lg) < 4 * pairs / nbuckets) ++lg;
-  return lg;
+  uint32_t heavy = 6;
+  while (heavy < 8 && (pairs >> (20 + heavy)) > 1) ++heavy;
+  return lg | heavy << 8;
 }
 // upper bound of the segments of `pairs` entries over `nbuckets` buckets
-// (a bucket of c entries has at most c / 2^MSM_HEAVY_SEG_LOG + 1)
-uint64_t msm_max_segments(uint64_t pairs, uint64_t nbuckets) { return (pairs >> MSM_HEAVY_SEG_LOG) + nbuckets; }
+// (a bucket of c entries has at most c / 2^heavy + 1)
+uint64_t msm_max_segments(uint64_t pairs, uint64_t nbuckets, uint32_t seg_code) {
+  return (pairs >> ((seg_code >> 8) & 0xFFu)) + nbuckets;
+}
 uint32_t msm_lo_bits(uint32_t c) { return std::min(8u, c - 1); }
 hipError_t launch_msm_buckets(const void* blimbs, const uint32_t* offsets, const uint32_t* counts,
                               const uint32_t* list, const uint32_t* part_seg_off, const uint32_t* seg_local,
-                              uint32_t nparts, uint32_t lo_bits, uint32_t seg_log, uint32_t nbuckets,
+                              uint32_t nparts, uint32_t lo_bits, uint32_t seg_code, uint32_t nbuckets,
                               uint64_t max_segs, void* seg_sums, uint32_t* seg_bucket, hipStream_t s) {
   msm_buckets_kernel<<<(uint32_t)((max_segs + 63) / 64), 64, 0, s>>>(
-      (const BaseLimbs*)blimbs, offsets, counts, list, part_seg_off, seg_local, nparts, lo_bits, seg_log, nbuckets,
+      (const BaseLimbs*)blimbs, offsets, counts, list, part_seg_off, seg_local, nparts, lo_bits, seg_code, nbuckets,
       (PointLimbs*)seg_sums, seg_bucket);
   return hipGetLastError();
 }
@@ -702,14 +710,15 @@ hipError_t launch_msm_buckets(const void* blimbs, const uint32_t* offsets, const
 // of n entries could need (idle ones exit on seg_max), then the gather.
 hipError_t launch_msm_bucket_merge(void* seg_sums, const uint32_t* seg_bucket, const uint32_t* counts,
                                    const uint32_t* part_seg_off, const uint32_t* seg_local, const uint32_t* seg_max,
-                                   uint32_t nparts, uint32_t lo_bits, uint32_t seg_log, uint32_t n, uint32_t nbuckets,
+                                   uint32_t nparts, uint32_t lo_bits, uint32_t seg_code, uint32_t n, uint32_t nbuckets,
                                    uint64_t max_segs, void* buckets, hipStream_t s) {
-  // segments of a bucket of at most n entries: two, or n / 2^8 for a heavy one
-  const uint64_t most = std::max<uint64_t>(2, ((uint64_t)n + (1u << MSM_HEAVY_SEG_LOG) - 1) >> MSM_HEAVY_SEG_LOG);
+  // segments of a bucket of at most n entries: two, or n / 2^heavy for a heavy one
+  const uint32_t heavy = (seg_code >> 8) & 0xFFu;
+  const uint64_t most = std::max<uint64_t>(2, ((uint64_t)n + (1u << heavy) - 1) >> heavy);
   const uint32_t blocks = (uint32_t)std::min<uint64_t>(1024, (max_segs + 255) / 256);
   for (uint32_t level = 0; (1ull << level) < most; ++level)
     msm_segment_tree_kernel<<<blocks, 256, 0, s>>>((PointLimbs*)seg_sums, seg_bucket, counts, part_seg_off, seg_local,
-                                                   seg_max, nparts, lo_bits, seg_log, level);
+                                                   seg_max, nparts, lo_bits, seg_code, level);
   msm_bucket_gather_kernel<<<(nbuckets + 63) / 64, 64, 0, s>>>((const PointLimbs*)seg_sums, counts, part_seg_off,
                                                                seg_local, lo_bits, nbuckets, (PointLimbs*)buckets);
   return hipGetLastError();
