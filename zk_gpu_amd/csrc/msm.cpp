@@ -214,7 +214,11 @@ MsmPlan msm_plan(uint32_t n) {
   p.c = lg < 23 ? kWidth[lg] : 16u;
   p.windows = (255 + p.c - 1) / p.c;  // signed digits: W * c >= 255 bits cover every scalar below 2^254
   p.half = 1u << (p.c - 1);
-  p.run = std::min(16u, p.half);
+  // runs of buckets per running-sum thread pair: the kernel is latency-bound
+  // (two additions per bucket in a chain), so runs are as short as the
+  // window tree allows: about 2^(c-1) / 256 buckets (two parts per run for
+  // each of the window kernel's 512 threads), between 4 and 16
+  p.run = std::min(p.half, std::max(4u, std::min(16u, p.half / 256)));
   p.runs = p.half / p.run;
   const size_t nb = (size_t)p.windows * p.half;
   size_t off = 0;
