@@ -28,10 +28,16 @@ hipError_t launch_msm_sort(const uint16_t* keys, uint32_t n, uint32_t c, uint32_
 uint32_t msm_lo_bits(uint32_t c);
 uint32_t msm_seg_code(uint64_t pairs, uint64_t nbuckets);
 uint64_t msm_max_segments(uint64_t pairs, uint64_t nbuckets, uint32_t seg_code);
+hipError_t launch_msm_seg_order(const uint32_t* part_seg_off, const uint32_t* seg_local, const uint32_t* counts,
+                                uint32_t nparts, uint32_t lo_bits, uint32_t seg_code, uint32_t nbuckets,
+                                uint64_t max_segs, uint32_t* seg_hist, uint32_t* bin_cursor, uint32_t* order_seg,
+                                uint32_t* order_bucket, hipStream_t s);
+uint32_t msm_seg_bins();
 hipError_t launch_msm_buckets(const void* blimbs, const uint32_t* offsets, const uint32_t* counts,
                               const uint32_t* list, const uint32_t* part_seg_off, const uint32_t* seg_local,
                               uint32_t nparts, uint32_t lo_bits, uint32_t seg_code, uint32_t nbuckets,
-                              uint64_t max_segs, void* seg_sums, uint32_t* seg_bucket, hipStream_t s);
+                              uint64_t max_segs, const uint32_t* order_seg, const uint32_t* order_bucket,
+                              void* seg_sums, uint32_t* seg_bucket, hipStream_t s);
 hipError_t launch_msm_bucket_merge(void* seg_sums, const uint32_t* seg_bucket, const uint32_t* counts,
                                    const uint32_t* part_seg_off, const uint32_t* seg_local, const uint32_t* seg_max,
                                    uint32_t nparts, uint32_t lo_bits, uint32_t seg_code, uint32_t n, uint32_t nbuckets,
@@ -194,7 +200,8 @@ struct MsmPlan {
   size_t off_blimbs = 0, off_keys = 0, off_tentry = 0, off_tlo = 0, off_list = 0, off_counts = 0, off_offsets = 0,
          off_pcounts = 0, off_poff = 0, off_pcursor = 0, off_slice_off = 0, off_cursor = 0, off_seg_local = 0,
          off_psegs = 0, off_pseg_off = 0,
-         off_seg_sums = 0, off_seg_bucket = 0, off_buckets = 0, off_parts = 0, off_wout = 0, bytes = 0;
+         off_seg_sums = 0, off_seg_bucket = 0, off_seg_hist = 0, off_bin_cursor = 0, off_order_seg = 0,
+         off_order_bucket = 0, off_buckets = 0, off_parts = 0, off_wout = 0, bytes = 0;
 };
 
 static size_t up256(size_t b) { return (b + 255) & ~(size_t)255; }
@@ -239,9 +246,12 @@ MsmPlan msm_plan(uint32_t n) {
   p.off_list = take(pairs * 4);
   const size_t m = (size_t)p.windows * p.sort_parts;  // partitions; the scans write m + 1 words
   // one zeroed region: partition sizes, the most segments of one bucket, the
-  // bucket counts
-  p.off_pcounts = take((m + 1 + nb) * 4);
+  // bucket counts, the segment-length histogram
+  const size_t bins = msm_seg_bins();
+  p.off_pcounts = take((m + 1 + nb + bins) * 4);
   p.off_counts = p.off_pcounts + (m + 1) * 4;
+  p.off_seg_hist = p.off_counts + nb * 4;
+  p.off_bin_cursor = take((bins + 1) * 4);
   p.off_offsets = take(nb * 4);
   p.off_cursor = take(nb * 4);
   p.off_poff = take((m + 1) * 4);
@@ -254,6 +264,8 @@ MsmPlan msm_plan(uint32_t n) {
   p.max_segs = msm_max_segments(pairs, nb, p.seg_code);
   p.off_seg_sums = take(p.max_segs * msm_point_limb_bytes());
   p.off_seg_bucket = take(p.max_segs * 4);
+  p.off_order_seg = take(p.max_segs * 4);
+  p.off_order_bucket = take(p.max_segs * 4);
   p.off_buckets = take(nb * msm_point_limb_bytes());
   p.off_parts = take((size_t)p.windows * p.runs * 2 * msm_point_limb_bytes());  // two parts per run
   p.off_wout = take((size_t)p.windows * 96);
@@ -290,7 +302,7 @@ int msm_enqueue(Engine* eng, const MsmPlan& p, const void* d_scalars, const void
   const uint32_t m = p.windows * p.sort_parts;
   uint32_t* seg_max = pcounts + m;
   uint32_t* seg_bucket = (uint32_t*)(b + p.off_seg_bucket);
-  step("reset_counters_kernel", [&] { return launch_reset_counters(pcounts, m + 1 + nb, s); });
+  step("reset_counters_kernel", [&] { return launch_reset_counters(pcounts, m + 1 + nb + msm_seg_bins(), s); });
   step("msm_digits_kernel", [&] {
     return launch_msm_digits(d_scalars, d_bases, p.n, p.c, p.windows, p.lo_bits, b + p.off_blimbs,
                              (uint16_t*)(b + p.off_keys), pcounts, s);
@@ -303,10 +315,17 @@ int msm_enqueue(Engine* eng, const MsmPlan& p, const void* d_scalars, const void
                            (uint32_t*)(b + p.off_list), p.seg_code, seg_local, (uint32_t*)(b + p.off_psegs), pseg_off,
                            seg_max, s);
   });
+  uint32_t* order_seg = (uint32_t*)(b + p.off_order_seg);
+  uint32_t* order_bucket = (uint32_t*)(b + p.off_order_bucket);
+  step("msm_seg_order (3 kernels)", [&] {
+    return launch_msm_seg_order(pseg_off, seg_local, counts, m, p.lo_bits, p.seg_code, nb, p.max_segs,
+                                (uint32_t*)(b + p.off_seg_hist), (uint32_t*)(b + p.off_bin_cursor), order_seg,
+                                order_bucket, s);
+  });
   step("msm_buckets_kernel", [&] {
     return launch_msm_buckets(b + p.off_blimbs, (const uint32_t*)(b + p.off_offsets), counts,
                               (const uint32_t*)(b + p.off_list), pseg_off, seg_local, m, p.lo_bits, p.seg_code, nb,
-                              p.max_segs, b + p.off_seg_sums, seg_bucket, s);
+                              p.max_segs, order_seg, order_bucket, b + p.off_seg_sums, seg_bucket, s);
   });
   step("msm_bucket_merge (tree levels, gather)", [&] {
     return launch_msm_bucket_merge(b + p.off_seg_sums, seg_bucket, counts, pseg_off, seg_local, seg_max, m,

@@ -448,23 +448,106 @@ __device__ __forceinline__ uint32_t seg_prefix(const uint32_t* part_seg_off, con
   return part_seg_off[b >> lo_bits] + seg_local[b];
 }
 
-// One thread per bucket segment (a run of one bucket's entries): the
-// sum of its points, accumulated in XYZZ coordinates, stored in Jacobian ones.
-// The segment's bucket is the largest b whose first segment is <= s (empty
-// buckets have no segments, so that bucket is never empty).
-__global__ void __launch_bounds__(64) msm_buckets_kernel(const BaseLimbs* __restrict__ blimbs, const uint32_t* __restrict__ offsets,
-                                   const uint32_t* __restrict__ counts, const uint32_t* __restrict__ list,
-                                   const uint32_t* __restrict__ part_seg_off, const uint32_t* __restrict__ seg_local,
-                                   uint32_t nparts, uint32_t lo_bits, uint32_t seg_code, uint32_t nbuckets,
-                                   PointLimbs* __restrict__ seg_sums, uint32_t* __restrict__ seg_bucket) {
-  const uint32_t sidx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (sidx >= part_seg_off[nparts]) return;
-  uint32_t lo = 0, hi = nbuckets;  // largest b with seg_prefix(b) <= sidx
+// The bucket of global segment sidx: the largest b whose first segment is
+// <= sidx (empty buckets have no segments, so that bucket is never empty).
+__device__ __forceinline__ uint32_t segment_bucket(const uint32_t* part_seg_off, const uint32_t* seg_local,
+                                                   uint32_t lo_bits, uint32_t nbuckets, uint32_t sidx) {
+  uint32_t lo = 0, hi = nbuckets;
   while (hi - lo > 1) {
     const uint32_t mid = (lo + hi) >> 1;
     if (seg_prefix(part_seg_off, seg_local, mid, lo_bits) <= sidx) lo = mid; else hi = mid;
   }
-  const uint32_t b = lo;
+  return lo;
+}
+
+// Segments are handed to the bucket-sum threads longest first, so that the
+// 64 lanes of a wave sum segments of about the same length (a wave runs as
+// long as its longest lane; with random digits the bucket sizes of one wave
+// spread over +-2.4 standard deviations) and the longest waves start first.
+// The order is a counting sort on SEG_BINS bins of the segment length
+// (shifted so that the longest ordinary segment, 2^(lg + 1) entries, fits).
+constexpr uint32_t SEG_BINS = 1024;
+__device__ __forceinline__ uint32_t segment_bin(uint32_t len, uint32_t seg_code) {
+  const uint32_t lg = seg_code & 0xFFu;
+  const uint32_t shift = lg + 1 > 10 ? lg + 1 - 10 : 0u;
+  return SEG_BINS - 1 - min(len >> shift, SEG_BINS - 1);  // longest first
+}
+__device__ __forceinline__ uint32_t segment_len(const uint32_t* part_seg_off, const uint32_t* seg_local,
+                                                const uint32_t* counts, uint32_t lo_bits, uint32_t seg_code,
+                                                uint32_t b, uint32_t sidx) {
+  const uint32_t count = counts[b], lg = bucket_seg_log(count, seg_code);
+  const uint32_t first = (sidx - seg_prefix(part_seg_off, seg_local, b, lo_bits)) << lg;
+  return min(1u << lg, count - first);
+}
+
+// One thread per segment: the per-bin segment counts (LDS histogram, one
+// global atomic per bin and workgroup; seg_hist zeroed by the caller).
+__global__ void __launch_bounds__(256) msm_seg_hist_kernel(const uint32_t* __restrict__ part_seg_off,
+                                                           const uint32_t* __restrict__ seg_local,
+                                                           const uint32_t* __restrict__ counts, uint32_t nparts,
+                                                           uint32_t lo_bits, uint32_t seg_code, uint32_t nbuckets,
+                                                           uint32_t* __restrict__ seg_hist) {
+  __shared__ uint32_t h[SEG_BINS];
+  const uint32_t total = part_seg_off[nparts];
+  if (blockIdx.x * blockDim.x >= total) return;
+  for (uint32_t i = threadIdx.x; i < SEG_BINS; i += blockDim.x) h[i] = 0;
+  __syncthreads();
+  const uint32_t sidx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (sidx < total) {
+    const uint32_t b = segment_bucket(part_seg_off, seg_local, lo_bits, nbuckets, sidx);
+    atomicAdd(&h[segment_bin(segment_len(part_seg_off, seg_local, counts, lo_bits, seg_code, b, sidx), seg_code)],
+              1u);
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < SEG_BINS; i += blockDim.x)
+    if (h[i]) atomicAdd(&seg_hist[i], h[i]);
+}
+
+// One thread per segment: its place in the order (bin offsets from the
+// scan of seg_hist, as cursors: one atomic per bin and workgroup reserves
+// the workgroup's places). Writes the order as (segment, bucket) pairs.
+__global__ void __launch_bounds__(256) msm_seg_order_kernel(const uint32_t* __restrict__ part_seg_off,
+                                                            const uint32_t* __restrict__ seg_local,
+                                                            const uint32_t* __restrict__ counts, uint32_t nparts,
+                                                            uint32_t lo_bits, uint32_t seg_code, uint32_t nbuckets,
+                                                            uint32_t* __restrict__ bin_cursor,
+                                                            uint32_t* __restrict__ order_seg,
+                                                            uint32_t* __restrict__ order_bucket) {
+  __shared__ uint32_t h[SEG_BINS], base[SEG_BINS];
+  const uint32_t total = part_seg_off[nparts];
+  if (blockIdx.x * blockDim.x >= total) return;
+  for (uint32_t i = threadIdx.x; i < SEG_BINS; i += blockDim.x) h[i] = 0;
+  __syncthreads();
+  const uint32_t sidx = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = sidx < total;
+  uint32_t b = 0, bin = 0, rank = 0;
+  if (valid) {
+    b = segment_bucket(part_seg_off, seg_local, lo_bits, nbuckets, sidx);
+    bin = segment_bin(segment_len(part_seg_off, seg_local, counts, lo_bits, seg_code, b, sidx), seg_code);
+    rank = atomicAdd(&h[bin], 1u);
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < SEG_BINS; i += blockDim.x) base[i] = h[i] ? atomicAdd(&bin_cursor[i], h[i]) : 0u;
+  __syncthreads();
+  if (valid) {
+    const uint32_t pos = base[bin] + rank;
+    order_seg[pos] = sidx;
+    order_bucket[pos] = b;
+  }
+}
+
+// One thread per bucket segment (a run of one bucket's entries), in the
+// order of msm_seg_order_kernel: the sum of its points, accumulated in XYZZ
+// coordinates, stored in Jacobian ones at the segment's index.
+__global__ void __launch_bounds__(64) msm_buckets_kernel(const BaseLimbs* __restrict__ blimbs, const uint32_t* __restrict__ offsets,
+                                   const uint32_t* __restrict__ counts, const uint32_t* __restrict__ list,
+                                   const uint32_t* __restrict__ part_seg_off, const uint32_t* __restrict__ seg_local,
+                                   uint32_t nparts, uint32_t lo_bits, uint32_t seg_code, uint32_t nbuckets,
+                                   const uint32_t* __restrict__ order_seg, const uint32_t* __restrict__ order_bucket,
+                                   PointLimbs* __restrict__ seg_sums, uint32_t* __restrict__ seg_bucket) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= part_seg_off[nparts]) return;
+  const uint32_t sidx = order_seg[t], b = order_bucket[t];
   seg_bucket[sidx] = b;
   const uint32_t count = counts[b], lg = bucket_seg_log(count, seg_code);
   const uint32_t first = (sidx - seg_prefix(part_seg_off, seg_local, b, lo_bits)) << lg;
@@ -697,13 +780,29 @@ uint64_t msm_max_segments(uint64_t pairs, uint64_t nbuckets, uint32_t seg_code) 
   return (pairs >> ((seg_code >> 8) & 0xFFu)) + nbuckets;
 }
 uint32_t msm_lo_bits(uint32_t c) { return std::min(8u, c - 1); }
+// The segments' order for the bucket sums (seg_hist, SEG_BINS words, zeroed
+// by the caller; bin_cursor SEG_BINS + 1 words; order_*: max_segs words).
+hipError_t launch_msm_seg_order(const uint32_t* part_seg_off, const uint32_t* seg_local, const uint32_t* counts,
+                                uint32_t nparts, uint32_t lo_bits, uint32_t seg_code, uint32_t nbuckets,
+                                uint64_t max_segs, uint32_t* seg_hist, uint32_t* bin_cursor, uint32_t* order_seg,
+                                uint32_t* order_bucket, hipStream_t s) {
+  const uint32_t blocks = (uint32_t)((max_segs + 255) / 256);
+  msm_seg_hist_kernel<<<blocks, 256, 0, s>>>(part_seg_off, seg_local, counts, nparts, lo_bits, seg_code, nbuckets,
+                                             seg_hist);
+  msm_part_scan_kernel<<<1, 256, 0, s>>>(seg_hist, SEG_BINS, bin_cursor, bin_cursor, nullptr);
+  msm_seg_order_kernel<<<blocks, 256, 0, s>>>(part_seg_off, seg_local, counts, nparts, lo_bits, seg_code, nbuckets,
+                                              bin_cursor, order_seg, order_bucket);
+  return hipGetLastError();
+}
+uint32_t msm_seg_bins() { return SEG_BINS; }
 hipError_t launch_msm_buckets(const void* blimbs, const uint32_t* offsets, const uint32_t* counts,
                               const uint32_t* list, const uint32_t* part_seg_off, const uint32_t* seg_local,
                               uint32_t nparts, uint32_t lo_bits, uint32_t seg_code, uint32_t nbuckets,
-                              uint64_t max_segs, void* seg_sums, uint32_t* seg_bucket, hipStream_t s) {
+                              uint64_t max_segs, const uint32_t* order_seg, const uint32_t* order_bucket,
+                              void* seg_sums, uint32_t* seg_bucket, hipStream_t s) {
   msm_buckets_kernel<<<(uint32_t)((max_segs + 63) / 64), 64, 0, s>>>(
       (const BaseLimbs*)blimbs, offsets, counts, list, part_seg_off, seg_local, nparts, lo_bits, seg_code, nbuckets,
-      (PointLimbs*)seg_sums, seg_bucket);
+      order_seg, order_bucket, (PointLimbs*)seg_sums, seg_bucket);
   return hipGetLastError();
 }
 // The segment sums reduced to one sum per bucket: the tree levels a bucket
