@@ -157,3 +157,24 @@ def test_job_struct_layout():
     assert Z.Job.omega.offset == 32
     assert ctypes.sizeof(Z.ServerResult) == 4 + 4 + 8 + 4 * 8 + 4 + 4
     assert ctypes.sizeof(Z.ServerStats) == 11 * 8
+
+
+def test_msm_plan_widths_and_workspace():
+    """The MSM plan is host arithmetic: window widths by size (a measured
+    table, 16 from 2^20 points), workspace growing with the length, nothing
+    for 0 or more than 2^28 points; above 2^27 points the workspace is that
+    of one 2^27-point piece."""
+    f = Z.lib().zkgpu_msm_workspace_bytes
+    c = ctypes.c_uint32(0)
+    want = {1: 4, 1 << 8: 5, 1 << 12: 9, 1 << 14: 10, 1 << 16: 12, 1 << 17: 15, 1 << 19: 15, 1 << 20: 16,
+            1 << 24: 16, 1 << 28: 16}
+    prev = 0
+    for n, cw in sorted(want.items()):
+        ws = f(n, ctypes.byref(c))
+        assert c.value == cw, n
+        assert ws > prev, n
+        prev = ws
+    assert f(1 << 28, None) == f(1 << 27, None)
+    assert f((1 << 27) + 1, None) <= f(1 << 27, None)
+    assert f(0, None) == 0
+    assert f((1 << 28) + 1, None) == 0
