@@ -37,7 +37,7 @@ hipError_t launch_msm_buckets(const void* blimbs, const uint32_t* offsets, const
                               const uint32_t* list, const uint32_t* part_seg_off, const uint32_t* seg_local,
                               uint32_t nparts, uint32_t lo_bits, uint32_t seg_code, uint32_t nbuckets,
                               uint64_t max_segs, const uint32_t* order_seg, const uint32_t* order_bucket,
-                              void* seg_sums, uint32_t* seg_bucket, hipStream_t s);
+                              void* seg_sums, uint32_t* seg_bucket, void* buckets, hipStream_t s);
 hipError_t launch_msm_bucket_merge(void* seg_sums, const uint32_t* seg_bucket, const uint32_t* counts,
                                    const uint32_t* part_seg_off, const uint32_t* seg_local, const uint32_t* seg_max,
                                    uint32_t nparts, uint32_t lo_bits, uint32_t seg_code, uint32_t n, uint32_t nbuckets,
@@ -325,7 +325,8 @@ int msm_enqueue(Engine* eng, const MsmPlan& p, const void* d_scalars, const void
   step("msm_buckets_kernel", [&] {
     return launch_msm_buckets(b + p.off_blimbs, (const uint32_t*)(b + p.off_offsets), counts,
                               (const uint32_t*)(b + p.off_list), pseg_off, seg_local, m, p.lo_bits, p.seg_code, nb,
-                              p.max_segs, order_seg, order_bucket, b + p.off_seg_sums, seg_bucket, s);
+                              p.max_segs, order_seg, order_bucket, b + p.off_seg_sums, seg_bucket,
+                              b + p.off_buckets, s);
   });
   step("msm_bucket_merge (tree levels, gather)", [&] {
     return launch_msm_bucket_merge(b + p.off_seg_sums, seg_bucket, counts, pseg_off, seg_local, seg_max, m,

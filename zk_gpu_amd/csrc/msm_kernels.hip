@@ -544,7 +544,8 @@ __global__ void __launch_bounds__(64) msm_buckets_kernel(const BaseLimbs* __rest
                                    const uint32_t* __restrict__ part_seg_off, const uint32_t* __restrict__ seg_local,
                                    uint32_t nparts, uint32_t lo_bits, uint32_t seg_code, uint32_t nbuckets,
                                    const uint32_t* __restrict__ order_seg, const uint32_t* __restrict__ order_bucket,
-                                   PointLimbs* __restrict__ seg_sums, uint32_t* __restrict__ seg_bucket) {
+                                   PointLimbs* __restrict__ seg_sums, uint32_t* __restrict__ seg_bucket,
+                                   PointLimbs* __restrict__ buckets) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= part_seg_off[nparts]) return;
   const uint32_t sidx = order_seg[t], b = order_bucket[t];
@@ -572,7 +573,12 @@ __global__ void __launch_bounds__(64) msm_buckets_kernel(const BaseLimbs* __rest
     e_next = list[min(start + k + 2, last)];
     g1_bucket_add(acc, inf, cur, neg);
   }
-  store_point(seg_sums, sidx, inf ? g1_identity() : g1_xyzz_to_jacobian(acc));
+  // a bucket of one segment (nearly every bucket with random scalars) is
+  // complete: its sum goes to the bucket array; the others' segment sums are
+  // merged by the tree and copied by the gather
+  const g1j sum = inf ? g1_identity() : g1_xyzz_to_jacobian(acc);
+  if (cnt == count) store_point(buckets, b, sum);
+  else store_point(seg_sums, sidx, sum);
 }
 
 // One level of the pairwise reduction of each bucket's segment sums: the
@@ -599,16 +605,19 @@ __global__ void __launch_bounds__(256) msm_segment_tree_kernel(
   }
 }
 
-// One thread per bucket: its sum (the first segment's, after the reduction)
-// or the identity for an empty bucket.
+// One thread per bucket: the sum of a bucket of several segments (its first
+// segment's, after the reduction) or the identity for an empty bucket; a
+// bucket of one segment has its sum from the bucket kernel.
 __global__ void msm_bucket_gather_kernel(const PointLimbs* __restrict__ seg_sums, const uint32_t* __restrict__ counts,
                                          const uint32_t* __restrict__ part_seg_off,
-                                         const uint32_t* __restrict__ seg_local, uint32_t lo_bits, uint32_t nbuckets,
-                                         PointLimbs* __restrict__ buckets) {
+                                         const uint32_t* __restrict__ seg_local, uint32_t lo_bits, uint32_t seg_code,
+                                         uint32_t nbuckets, PointLimbs* __restrict__ buckets) {
   const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nbuckets) return;
-  store_point(buckets, b,
-              counts[b] ? load_point(seg_sums, seg_prefix(part_seg_off, seg_local, b, lo_bits)) : g1_identity());
+  const uint32_t segs = bucket_segs(counts[b], seg_code);
+  if (segs == 0) store_point(buckets, b, g1_identity());
+  else if (segs > 1) store_point(buckets, b, load_point(seg_sums, seg_prefix(part_seg_off, seg_local, b, lo_bits)));
+  // one segment: the bucket kernel stored the sum
 }
 
 // Two threads, in different waves, per run of `run` buckets [lo, lo + run)
@@ -799,10 +808,10 @@ hipError_t launch_msm_buckets(const void* blimbs, const uint32_t* offsets, const
                               const uint32_t* list, const uint32_t* part_seg_off, const uint32_t* seg_local,
                               uint32_t nparts, uint32_t lo_bits, uint32_t seg_code, uint32_t nbuckets,
                               uint64_t max_segs, const uint32_t* order_seg, const uint32_t* order_bucket,
-                              void* seg_sums, uint32_t* seg_bucket, hipStream_t s) {
+                              void* seg_sums, uint32_t* seg_bucket, void* buckets, hipStream_t s) {
   msm_buckets_kernel<<<(uint32_t)((max_segs + 63) / 64), 64, 0, s>>>(
       (const BaseLimbs*)blimbs, offsets, counts, list, part_seg_off, seg_local, nparts, lo_bits, seg_code, nbuckets,
-      order_seg, order_bucket, (PointLimbs*)seg_sums, seg_bucket);
+      order_seg, order_bucket, (PointLimbs*)seg_sums, seg_bucket, (PointLimbs*)buckets);
   return hipGetLastError();
 }
 // The segment sums reduced to one sum per bucket: the tree levels a bucket
@@ -819,7 +828,8 @@ hipError_t launch_msm_bucket_merge(void* seg_sums, const uint32_t* seg_bucket, c
     msm_segment_tree_kernel<<<blocks, 256, 0, s>>>((PointLimbs*)seg_sums, seg_bucket, counts, part_seg_off, seg_local,
                                                    seg_max, nparts, lo_bits, seg_code, level);
   msm_bucket_gather_kernel<<<(nbuckets + 63) / 64, 64, 0, s>>>((const PointLimbs*)seg_sums, counts, part_seg_off,
-                                                               seg_local, lo_bits, nbuckets, (PointLimbs*)buckets);
+                                                               seg_local, lo_bits, seg_code, nbuckets,
+                                                               (PointLimbs*)buckets);
   return hipGetLastError();
 }
 // parts: 2 x runs_per_window x windows point entries
