@@ -146,6 +146,28 @@ def _mix64(z):
     return z ^ (z >> 31)
 
 
+def test_repeated_calls_identical(torch_cuda):
+    """The order of a bucket's entries follows the sort's atomic
+    reservations and varies from call to call; the result, a group element
+    returned in affine form, does not."""
+    torch = torch_cuda
+    n = (1 << 20) + 77
+    ds = torch.empty(n * 4, dtype=torch.int64, device="cuda")
+    db = torch.empty(n * 8, dtype=torch.int64, device="cuda")
+    Z.fill_random(ptr(ds), n, 515)
+    Z.fill_random_points(ptr(db), n, 616, 0)
+    torch.cuda.synchronize()
+    first = np.asarray(Z.msm_device(ptr(ds), ptr(db), n))
+    for _ in range(3):
+        assert (np.asarray(Z.msm_device(ptr(ds), ptr(db), n)) == first).all()
+    # all scalars equal: one bucket per window, summed in segments and merged
+    ds.view(n, 4)[:] = ds.view(n, 4)[0].clone()
+    torch.cuda.synchronize()
+    first = np.asarray(Z.msm_device(ptr(ds), ptr(db), n))
+    for _ in range(3):
+        assert (np.asarray(Z.msm_device(ptr(ds), ptr(db), n)) == first).all()
+
+
 def _splitmix_sum(n, seed, start=0):
     """sum of the synthetic base multipliers k_i (include/zkgpu_ntt.h:
     SplitMix64 of (seed, start + i), low bit set), as an integer"""
