@@ -168,6 +168,34 @@ def test_repeated_calls_identical(torch_cuda):
         assert (np.asarray(Z.msm_device(ptr(ds), ptr(db), n)) == first).all()
 
 
+def test_concurrent_callers():
+    """Six host threads call the host-buffer MSM at once (more than the four
+    host-call contexts, so some wait for one); every result equals the
+    closed form of its own inputs."""
+    import threading
+    cases = []
+    for t in range(6):
+        n = 3000 + 977 * t
+        a, d = 1000 + t, 77 + 13 * t
+        cases.append((O.gen(4800 + t, n), O.g1_arith_bases(n, a, d), a, d))
+    got = [None] * len(cases)
+
+    def worker(t):
+        sc, bs, _, _ = cases[t]
+        acc = np.zeros((3, 4), dtype=np.uint64)
+        st = Z.parallel_msm_api(0, sc, bs, sc.shape[0], acc)
+        got[t] = (st, _affine(acc) if st == 0 else None)
+
+    ts = [threading.Thread(target=worker, args=(t,)) for t in range(len(cases))]
+    for th in ts:
+        th.start()
+    for th in ts:
+        th.join()
+    for t, (sc, _, a, d) in enumerate(cases):
+        assert got[t][0] == 0, t
+        assert got[t][1] == O.g1_mul_gen(_weighted_sum(sc, a, d)), t
+
+
 def _splitmix_sum(n, seed, start=0):
     """sum of the synthetic base multipliers k_i (include/zkgpu_ntt.h:
     SplitMix64 of (seed, start + i), low bit set), as an integer"""
