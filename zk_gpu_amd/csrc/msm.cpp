@@ -23,9 +23,11 @@ hipError_t launch_msm_digits(const void* scalars, const void* bases, uint32_t n,
 hipError_t launch_msm_sort(const uint16_t* keys, uint32_t n, uint32_t c, uint32_t windows, uint32_t lo_bits,
                            const uint32_t* part_counts, uint32_t* part_off, uint32_t* part_cursor,
                            uint32_t* slice_off, uint32_t* t_entry, uint8_t* t_lo, uint32_t* counts, uint32_t* offsets,
-                           uint32_t* cursor, uint32_t* list, uint32_t seg_code, uint32_t* seg_local,
-                           uint32_t* part_segs, uint32_t* part_seg_off, uint32_t* seg_max, hipStream_t s);
+                           uint32_t* cursor, uint32_t* list, uint32_t* slice_hist, uint32_t seg_code,
+                           uint32_t* seg_local, uint32_t* part_segs, uint32_t* part_seg_off, uint32_t* seg_max,
+                           hipStream_t s);
 uint32_t msm_lo_bits(uint32_t c);
+uint64_t msm_max_slices(uint64_t pairs, uint32_t m);
 uint32_t msm_seg_code(uint64_t pairs, uint64_t nbuckets);
 uint64_t msm_max_segments(uint64_t pairs, uint64_t nbuckets, uint32_t seg_code);
 hipError_t launch_msm_seg_order(const uint32_t* part_seg_off, const uint32_t* seg_local, const uint32_t* counts,
@@ -198,7 +200,8 @@ struct MsmPlan {
   uint32_t seg_code = 0;  // bucket segment lengths (ordinary and heavy buckets), packed for the kernels
   uint64_t max_segs = 0;  // bound on their number
   size_t off_blimbs = 0, off_keys = 0, off_tentry = 0, off_tlo = 0, off_list = 0, off_counts = 0, off_offsets = 0,
-         off_pcounts = 0, off_poff = 0, off_pcursor = 0, off_slice_off = 0, off_cursor = 0, off_seg_local = 0,
+         off_pcounts = 0, off_poff = 0, off_pcursor = 0, off_slice_off = 0, off_slice_hist = 0, off_cursor = 0,
+         off_seg_local = 0,
          off_psegs = 0, off_pseg_off = 0,
          off_seg_sums = 0, off_seg_bucket = 0, off_seg_hist = 0, off_bin_cursor = 0, off_order_seg = 0,
          off_order_bucket = 0, off_buckets = 0, off_parts = 0, off_wout = 0, bytes = 0;
@@ -257,6 +260,7 @@ MsmPlan msm_plan(uint32_t n) {
   p.off_poff = take((m + 1) * 4);
   p.off_pcursor = take((m + 1) * 4);
   p.off_slice_off = take((m + 1) * 4);
+  p.off_slice_hist = take(msm_max_slices(pairs, (uint32_t)m) * ((size_t)1 << p.lo_bits) * 4);
   p.off_seg_local = take(nb * 4);
   p.off_psegs = take(m * 4);
   p.off_pseg_off = take((m + 1) * 4);
@@ -312,8 +316,8 @@ int msm_enqueue(Engine* eng, const MsmPlan& p, const void* d_scalars, const void
                            (uint32_t*)(b + p.off_poff), (uint32_t*)(b + p.off_pcursor),
                            (uint32_t*)(b + p.off_slice_off), (uint32_t*)(b + p.off_tentry), (uint8_t*)(b + p.off_tlo),
                            counts, (uint32_t*)(b + p.off_offsets), (uint32_t*)(b + p.off_cursor),
-                           (uint32_t*)(b + p.off_list), p.seg_code, seg_local, (uint32_t*)(b + p.off_psegs), pseg_off,
-                           seg_max, s);
+                           (uint32_t*)(b + p.off_list), (uint32_t*)(b + p.off_slice_hist), p.seg_code, seg_local,
+                           (uint32_t*)(b + p.off_psegs), pseg_off, seg_max, s);
   });
   uint32_t* order_seg = (uint32_t*)(b + p.off_order_seg);
   uint32_t* order_bucket = (uint32_t*)(b + p.off_order_bucket);

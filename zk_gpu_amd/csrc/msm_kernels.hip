@@ -326,13 +326,14 @@ __device__ __forceinline__ bool sort_slice(const uint32_t* slice_off, const uint
   return true;
 }
 
-// One workgroup per slice: the slice's bucket counts in LDS, added to the
-// global counts (bucket g * 2^lo_bits + low bits for partition g; zeroed by
-// the caller).
+// One workgroup per slice: the slice's bucket counts in LDS, kept for the
+// scatter (slice_hist, 2^lo_bits words per slice) and added to the global
+// counts (bucket g * 2^lo_bits + low bits for partition g; zeroed by the
+// caller).
 __global__ void __launch_bounds__(SORT_THREADS) msm_bucket_count_kernel(
     const uint32_t* __restrict__ slice_off, const uint32_t* __restrict__ part_off,
     const uint32_t* __restrict__ part_counts, uint32_t m, const uint8_t* __restrict__ t_lo, uint32_t lo_bits,
-    uint32_t* __restrict__ counts) {
+    uint32_t* __restrict__ counts, uint32_t* __restrict__ slice_hist) {
   __shared__ uint32_t hist[256];
   uint32_t g, start, end;
   if (!sort_slice(slice_off, part_off, part_counts, m, blockIdx.x, &g, &start, &end)) return;
@@ -341,8 +342,10 @@ __global__ void __launch_bounds__(SORT_THREADS) msm_bucket_count_kernel(
   __syncthreads();
   for (uint32_t j = start + threadIdx.x; j < end; j += SORT_THREADS) atomicAdd(&hist[t_lo[j]], 1u);
   __syncthreads();
-  for (uint32_t l = threadIdx.x; l < nlo; l += SORT_THREADS)
+  for (uint32_t l = threadIdx.x; l < nlo; l += SORT_THREADS) {
+    slice_hist[(blockIdx.x << lo_bits) + l] = hist[l];
     if (hist[l]) atomicAdd(&counts[(g << lo_bits) + l], hist[l]);
+  }
 }
 
 // One workgroup per partition: its buckets' list offsets (and the scatter's
@@ -378,26 +381,27 @@ __global__ void __launch_bounds__(SORT_THREADS) msm_bucket_offsets_kernel(
   }
 }
 
-// One workgroup per slice: its bucket counts again, one atomic per bucket on
-// the cursors reserves the slice's run of each bucket, then the slice goes
+// One workgroup per slice: with the slice's bucket counts (slice_hist), one
+// atomic per bucket on the cursors reserves the slice's run of each bucket,
+// then the slice goes
 // into those runs a SORT_CHUNK chunk at a time, sorted by bucket in LDS so
 // that consecutive threads write each run.
 __global__ void __launch_bounds__(SORT_THREADS) msm_bucket_scatter_kernel(
     const uint32_t* __restrict__ slice_off, const uint32_t* __restrict__ part_off,
     const uint32_t* __restrict__ part_counts, uint32_t m, const uint32_t* __restrict__ t_entry,
-    const uint8_t* __restrict__ t_lo, uint32_t lo_bits, uint32_t* __restrict__ cursor, uint32_t* __restrict__ list) {
+    const uint8_t* __restrict__ t_lo, uint32_t lo_bits, const uint32_t* __restrict__ slice_hist,
+    uint32_t* __restrict__ cursor, uint32_t* __restrict__ list) {
   __shared__ uint32_t cur[256], lcnt[256], loff[256];
   __shared__ uint32_t st_entry[SORT_CHUNK];
   __shared__ uint8_t st_lo[SORT_CHUNK];
   uint32_t g, start, end;
   if (!sort_slice(slice_off, part_off, part_counts, m, blockIdx.x, &g, &start, &end)) return;
   const uint32_t nlo = 1u << lo_bits;
-  for (uint32_t l = threadIdx.x; l < nlo; l += SORT_THREADS) lcnt[l] = 0;
-  __syncthreads();
-  for (uint32_t j = start + threadIdx.x; j < end; j += SORT_THREADS) atomicAdd(&lcnt[t_lo[j]], 1u);
-  __syncthreads();
-  for (uint32_t l = threadIdx.x; l < nlo; l += SORT_THREADS)
-    cur[l] = lcnt[l] ? atomicAdd(&cursor[(g << lo_bits) + l], lcnt[l]) : 0u;
+  // the slice's bucket counts, from the count kernel
+  for (uint32_t l = threadIdx.x; l < nlo; l += SORT_THREADS) {
+    const uint32_t h = slice_hist[(blockIdx.x << lo_bits) + l];
+    cur[l] = h ? atomicAdd(&cursor[(g << lo_bits) + l], h) : 0u;
+  }
   for (uint32_t c0 = start; c0 < end; c0 += SORT_CHUNK) {
     const uint32_t len = min(SORT_CHUNK, end - c0);
     __syncthreads();  // cur is set; the previous chunk's reads of lcnt / loff / staging are done
@@ -755,8 +759,9 @@ uint64_t msm_max_slices(uint64_t pairs, uint32_t m) { return (pairs >> SORT_SLIC
 hipError_t launch_msm_sort(const uint16_t* keys, uint32_t n, uint32_t c, uint32_t windows, uint32_t lo_bits,
                            const uint32_t* part_counts, uint32_t* part_off, uint32_t* part_cursor,
                            uint32_t* slice_off, uint32_t* t_entry, uint8_t* t_lo, uint32_t* counts, uint32_t* offsets,
-                           uint32_t* cursor, uint32_t* list, uint32_t seg_code, uint32_t* seg_local,
-                           uint32_t* part_segs, uint32_t* part_seg_off, uint32_t* seg_max, hipStream_t s) {
+                           uint32_t* cursor, uint32_t* list, uint32_t* slice_hist, uint32_t seg_code,
+                           uint32_t* seg_local, uint32_t* part_segs, uint32_t* part_seg_off, uint32_t* seg_max,
+                           hipStream_t s) {
   if (c < 2 || c > 16 || lo_bits > 8 || lo_bits > c - 1) return hipErrorInvalidValue;
   const uint32_t parts = 1u << (c - 1 - lo_bits), m = windows * parts;
   if (parts > MSM_MAX_PARTS || m > MSM_MAX_COUNTERS) return hipErrorInvalidValue;
@@ -765,12 +770,12 @@ hipError_t launch_msm_sort(const uint16_t* keys, uint32_t n, uint32_t c, uint32_
   msm_partition_kernel<<<dim3((n + SORT_CHUNK - 1) / SORT_CHUNK, windows), SORT_THREADS, 0, s>>>(
       keys, n, c, lo_bits, part_cursor, t_entry, t_lo);
   msm_bucket_count_kernel<<<slices, SORT_THREADS, 0, s>>>(slice_off, part_off, part_counts, m, t_lo, lo_bits,
-                                                          counts);
+                                                          counts, slice_hist);
   msm_bucket_offsets_kernel<<<m, SORT_THREADS, 0, s>>>(part_off, counts, lo_bits, seg_code, offsets, cursor,
                                                        seg_local, part_segs, seg_max);
   msm_part_scan_kernel<<<1, 256, 0, s>>>(part_segs, m, part_seg_off, part_seg_off, nullptr);
   msm_bucket_scatter_kernel<<<slices, SORT_THREADS, 0, s>>>(slice_off, part_off, part_counts, m, t_entry, t_lo,
-                                                            lo_bits, cursor, list);
+                                                            lo_bits, slice_hist, cursor, list);
   return hipGetLastError();
 }
 // Segment lengths (bucket_seg_log): ordinary 2^8..2^12 entries, four times
