@@ -161,13 +161,13 @@ def test_job_struct_layout():
 
 def test_msm_plan_widths_and_workspace():
     """The MSM plan is host arithmetic: window widths by size (a measured
-    table, 16 from 2^20 points), workspace growing with the length, nothing
+    table, 15 from 2^14 points, 16 from 2^20), workspace growing with the length, nothing
     for 0 or more than 2^28 points; above 2^27 points the workspace is that
     of one 2^27-point piece."""
     f = Z.lib().zkgpu_msm_workspace_bytes
     c = ctypes.c_uint32(0)
-    want = {1: 4, 1 << 8: 5, 1 << 12: 9, 1 << 14: 10, 1 << 16: 12, 1 << 17: 15, 1 << 19: 15, 1 << 20: 16,
-            1 << 24: 16, 1 << 28: 16}
+    want = {1: 4, 1 << 7: 5, 1 << 8: 8, 1 << 12: 10, 1 << 13: 11, 1 << 14: 15, 1 << 16: 15, 1 << 19: 15,
+            1 << 20: 16, 1 << 24: 16, 1 << 28: 16}
     prev = 0
     for n, cw in sorted(want.items()):
         ws = f(n, ctypes.byref(c))

@@ -214,13 +214,14 @@ MsmPlan msm_plan(uint32_t n) {
   p.n = n;
   uint32_t lg = 0;
   while ((1ull << (lg + 1)) <= n) ++lg;
-  // window width by floor(log2 n), measured on gfx950 (every c from
-  // log2 n - 8 to log2 n - 2 at 2^12..2^22 random points): below 2^20 points
-  // the latency-bound running sums and window sums favour wide windows, and
-  // widths whose top window holds only one or two bits of the 254-bit
-  // scalars (c = 11, 13, 14) lose; log2 n - 3 below 2^12
-  static const uint8_t kWidth[23] = {4, 4, 4, 4, 4, 4, 4, 4, 5, 6, 7, 8,                 // 2^0 .. 2^11
-                                     9, 9, 10, 12, 12, 15, 15, 15, 16, 16, 16};  // 2^12 .. 2^22
+  // window width by floor(log2 n), measured on gfx950 (sweeps of c = 4..16
+  // at 2^6..2^22 random points): up to 2^19 points the latency-bound
+  // running sums and window sums cost about the same at every width while
+  // the bucket chains shorten as the windows widen, so wide windows win
+  // (c = 15 from 2^14: W = 17 windows, the top one full); widths whose top
+  // window holds only a few bits of the 254-bit scalars lose
+  static const uint8_t kWidth[23] = {4,  4,  4,  4,  4,  4,  5,  5,  8,  8,  9,  10,   // 2^0 .. 2^11
+                                     10, 11, 15, 15, 15, 15, 15, 15, 16, 16, 16};  // 2^12 .. 2^22
   p.c = lg < 23 ? kWidth[lg] : 16u;
   p.windows = (255 + p.c - 1) / p.c;  // signed digits: W * c >= 255 bits cover every scalar below 2^254
   p.half = 1u << (p.c - 1);

@@ -470,7 +470,7 @@ __device__ __forceinline__ uint32_t segment_bucket(const uint32_t* part_seg_off,
 // spread over +-2.4 standard deviations) and the longest waves start first.
 // The order is a counting sort on SEG_BINS bins of the segment length
 // (shifted so that the longest ordinary segment, 2^(lg + 1) entries, fits).
-constexpr uint32_t SEG_BINS = 1024;
+constexpr uint32_t SEG_BINS = 1024;  // > 2^(lg + 1) for lg < 10; longer segments binned coarser
 __device__ __forceinline__ uint32_t segment_bin(uint32_t len, uint32_t seg_code) {
   const uint32_t lg = seg_code & 0xFFu;
   const uint32_t shift = lg + 1 > 10 ? lg + 1 - 10 : 0u;
@@ -778,15 +778,16 @@ hipError_t launch_msm_sort(const uint16_t* keys, uint32_t n, uint32_t c, uint32_
                                                             lo_bits, slice_hist, cursor, list);
   return hipGetLastError();
 }
-// Segment lengths (bucket_seg_log): ordinary 2^8..2^12 entries, four times
-// the mean bucket size; heavy 2^6 entries up to 2^26 (window, point) pairs,
-// then longer, up to 2^8 from 2^28 pairs.
+// Segment lengths (bucket_seg_log): ordinary 2^5..2^12 entries, four times
+// the mean bucket size; heavy 2^4 entries below 2^18 (window, point) pairs,
+// 2^6 up to 2^26 pairs, then longer, up to 2^8 from 2^28 pairs (never
+// longer than the ordinary ones).
 uint32_t msm_seg_code(uint64_t pairs, uint64_t nbuckets) {
-  uint32_t lg = 8;
+  uint32_t lg = 5;
   while (lg < 12 && (1ull << lg) < 4 * pairs / nbuckets) ++lg;
-  uint32_t heavy = 6;
+  uint32_t heavy = pairs < (1ull << 18) ? 4 : 6;
   while (heavy < 8 && (pairs >> (20 + heavy)) > 1) ++heavy;
-  return lg | heavy << 8;
+  return lg | std::min(heavy, lg) << 8;
 }
 // upper bound of the segments of `pairs` entries over `nbuckets` buckets
 // (a bucket of c entries has at most c / 2^heavy + 1)
