@@ -386,6 +386,36 @@ ZK_HD void g1_bucket_add(g1xyzz& acc, bool& inf, const g1a& q, bool neg) {
   acc.y = Y3;
 }
 
+ZK_HD g1xyzz g1_xyzz_identity() { return g1xyzz{fq::q_zero(), fq::q_zero(), fq::q_zero(), fq::q_zero()}; }
+// the identity has ZZ = 0 (and ZZZ = 0); every other point has ZZ != 0 mod q
+ZK_HD bool g1_xyzz_is_identity(const g1xyzz& a) { return fq::q_is_zero(a.zz); }
+
+// P1 + P2 in XYZZ coordinates, add-2008-s: 12M + 2S. Inputs and outputs:
+// X, Y normalized below 3q (q_sub's reduced results, or products);
+// ZZ, ZZZ normalized products below 1.1q. P == 0 means x1 == x2: then the
+// sum is 2 P1 (R == 0) or the identity.
+ZK_HD g1xyzz g1_xyzz_add(const g1xyzz& p, const g1xyzz& q) {
+  using namespace fq;
+  if (g1_xyzz_is_identity(p)) return q;
+  if (g1_xyzz_is_identity(q)) return p;
+  const fe U1 = q_mul(p.x, q.zz), U2 = q_mul(q.x, p.zz);
+  const fe S1 = q_mul(p.y, q.zzz), S2 = q_mul(q.y, p.zzz);
+  const fe P = q_sub(U2, U1), R = q_sub(S2, S1);
+  if (q_is_zero(P)) {
+    if (!q_is_zero(R)) return g1_xyzz_identity();
+    g1xyzz d = p;
+    g1_xyzz_double(d);
+    return d;
+  }
+  const fe PP = q_sqr(P), PPP = q_mul(P, PP), Q = q_mul(U1, PP);
+  g1xyzz o;
+  o.x = q_sub(q_sub(q_sqr(R), PPP), q_dbl(Q));
+  o.y = q_sub(q_mul(R, q_sub(Q, o.x)), q_mul(S1, PPP));
+  o.zz = q_mul(q_mul(p.zz, q.zz), PP);
+  o.zzz = q_mul(q_mul(p.zzz, q.zzz), PPP);
+  return o;
+}
+
 // P + Q, add-2007-bl: 11M + 5S
 ZK_HD g1j g1_add(const g1j& p, const g1j& q) {
   using namespace fq;

@@ -38,7 +38,8 @@ namespace zkgpu {
 struct BaseLimbs {
   uint4 x0, x1, y0, y1;
 };
-// a bucket / partial sum: Jacobian X, Y, Z in 48-byte entries
+// a bucket / partial sum: XYZZ coordinates X, Y, ZZ, ZZZ (nine limbs each) in
+// 144-byte entries
 struct PointLimbs {
   uint4 a[9];
 };
@@ -66,15 +67,38 @@ __device__ __forceinline__ g1a unpack_base(const BaseLimbs& e) {
   return p;
 }
 __device__ __forceinline__ g1a load_base(const BaseLimbs* b, uint32_t i) { return unpack_base(b[i]); }
-__device__ __forceinline__ g1j load_point(const PointLimbs* p, uint32_t i) {
+__device__ __forceinline__ g1xyzz load_point(const PointLimbs* p, uint32_t i) {
   const PointLimbs e = p[i];
-  return g1j{fe_from3(e.a[0], e.a[1], e.a[2]), fe_from3(e.a[3], e.a[4], e.a[5]), fe_from3(e.a[6], e.a[7], e.a[8])};
+  uint32_t w[36];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    w[4 * k] = e.a[k].x;
+    w[4 * k + 1] = e.a[k].y;
+    w[4 * k + 2] = e.a[k].z;
+    w[4 * k + 3] = e.a[k].w;
+  }
+  g1xyzz v;
+#pragma unroll
+  for (int k = 0; k < NLIMBS; ++k) {
+    v.x.l[k] = w[k];
+    v.y.l[k] = w[NLIMBS + k];
+    v.zz.l[k] = w[2 * NLIMBS + k];
+    v.zzz.l[k] = w[3 * NLIMBS + k];
+  }
+  return v;
 }
-__device__ __forceinline__ void store_point(PointLimbs* p, uint32_t i, const g1j& v) {
+__device__ __forceinline__ void store_point(PointLimbs* p, uint32_t i, const g1xyzz& v) {
+  uint32_t w[36];
+#pragma unroll
+  for (int k = 0; k < NLIMBS; ++k) {
+    w[k] = v.x.l[k];
+    w[NLIMBS + k] = v.y.l[k];
+    w[2 * NLIMBS + k] = v.zz.l[k];
+    w[3 * NLIMBS + k] = v.zzz.l[k];
+  }
   PointLimbs e;
-  fe_to3(v.x, e.a[0], e.a[1], e.a[2]);
-  fe_to3(v.y, e.a[3], e.a[4], e.a[5]);
-  fe_to3(v.z, e.a[6], e.a[7], e.a[8]);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) e.a[k] = make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]);
   p[i] = e;
 }
 
@@ -580,7 +604,7 @@ __global__ void __launch_bounds__(64) msm_buckets_kernel(const BaseLimbs* __rest
   // a bucket of one segment (nearly every bucket with random scalars) is
   // complete: its sum goes to the bucket array; the others' segment sums are
   // merged by the tree and copied by the gather
-  const g1j sum = inf ? g1_identity() : g1_xyzz_to_jacobian(acc);
+  const g1xyzz sum = inf ? g1_xyzz_identity() : acc;
   if (cnt == count) store_point(buckets, b, sum);
   else store_point(seg_sums, sidx, sum);
 }
@@ -605,7 +629,7 @@ __global__ void __launch_bounds__(256) msm_segment_tree_kernel(
     const uint32_t i = sidx - seg_prefix(part_seg_off, seg_local, b, lo_bits);
     if ((i & (2 * step - 1)) != 0) continue;
     if (i + step >= bucket_segs(counts[b], seg_code)) continue;
-    store_point(seg_sums, sidx, g1_add(load_point(seg_sums, sidx), load_point(seg_sums, sidx + step)));
+    store_point(seg_sums, sidx, g1_xyzz_add(load_point(seg_sums, sidx), load_point(seg_sums, sidx + step)));
   }
 }
 
@@ -619,7 +643,7 @@ __global__ void msm_bucket_gather_kernel(const PointLimbs* __restrict__ seg_sums
   const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nbuckets) return;
   const uint32_t segs = bucket_segs(counts[b], seg_code);
-  if (segs == 0) store_point(buckets, b, g1_identity());
+  if (segs == 0) store_point(buckets, b, g1_xyzz_identity());
   else if (segs > 1) store_point(buckets, b, load_point(seg_sums, seg_prefix(part_seg_off, seg_local, b, lo_bits)));
   // one segment: the bucket kernel stored the sum
 }
@@ -640,21 +664,22 @@ __global__ void __launch_bounds__(64) msm_chunks_kernel(const PointLimbs* __rest
   if (r >= total_runs) return;
   const uint32_t w = r / runs_per_window, k = r % runs_per_window;
   const uint32_t lo = k * run;
-  g1j S = g1_identity();
+  g1xyzz S = g1_xyzz_identity();
   if (role == 0) {
-    g1j T = g1_identity();
+    g1xyzz T = g1_xyzz_identity();
     for (uint32_t j = run; j-- > 0;) {
-      S = g1_add(S, load_point(buckets, w * half + lo + j));
-      T = g1_add(T, S);
+      S = g1_xyzz_add(S, load_point(buckets, w * half + lo + j));
+      T = g1_xyzz_add(T, S);
     }
     store_point(out, 2 * r, T);
   } else {
-    for (uint32_t j = run; j-- > 0;) S = g1_add(S, load_point(buckets, w * half + lo + j));
-    // lo * S by double-and-add, most significant bit first
-    g1j L = g1_identity();
-    for (int bit = lo ? 31 - __clz(lo) : -1; bit >= 0; --bit) {
-      L = g1_double(L);
-      if ((lo >> bit) & 1u) L = g1_add(L, S);
+    for (uint32_t j = run; j-- > 0;) S = g1_xyzz_add(S, load_point(buckets, w * half + lo + j));
+    // lo * S by double-and-add, most significant bit first (doubling the
+    // identity, ZZ = 0, gives ZZ = 0 again)
+    g1xyzz L = lo ? S : g1_xyzz_identity();
+    for (int bit = lo ? 30 - __clz(lo) : -1; bit >= 0; --bit) {
+      g1_xyzz_double(L);
+      if ((lo >> bit) & 1u) L = g1_xyzz_add(L, S);
     }
     store_point(out, 2 * r + 1, L);
   }
@@ -668,20 +693,21 @@ __global__ void __launch_bounds__(WINDOW_THREADS) msm_windows_kernel(const Point
                                                                      uint4* __restrict__ out) {
   __shared__ PointLimbs tree[WINDOW_THREADS];
   const uint32_t w = blockIdx.x;
-  g1j acc = g1_identity();
+  g1xyzz acc = g1_xyzz_identity();
   for (uint32_t k = threadIdx.x; k < parts_per_window; k += WINDOW_THREADS)
-    acc = g1_add(acc, load_point(parts, w * parts_per_window + k));
+    acc = g1_xyzz_add(acc, load_point(parts, w * parts_per_window + k));
   store_point(tree, threadIdx.x, acc);
   __syncthreads();
   for (uint32_t off = WINDOW_THREADS / 2; off > 0; off >>= 1) {
     if (threadIdx.x < off) {
-      const g1j a = g1_add(load_point(tree, threadIdx.x), load_point(tree, threadIdx.x + off));
+      const g1xyzz a = g1_xyzz_add(load_point(tree, threadIdx.x), load_point(tree, threadIdx.x + off));
       store_point(tree, threadIdx.x, a);
     }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    const g1j s = load_point(tree, 0);
+    // Jacobian (X ZZ^2, Y ZZZ^2, ZZZ); the identity (ZZ = ZZZ = 0) maps to Z = 0
+    const g1j s = g1_xyzz_to_jacobian(load_point(tree, 0));
     uint32_t x[8], y[8], z[8];
     fq::q_to_words(x, s.x);
     fq::q_to_words(y, s.y);
