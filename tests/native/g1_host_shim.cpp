@@ -27,7 +27,48 @@ static void jstore(uint64_t* o, const g1j& p) {
   to64(o + 8, p.z);
 }
 
+// Jacobian words (X, Y, Z) -> XYZZ (X, Y, Z^2, Z^3): x = X / Z^2, y = Y / Z^3
+static g1xyzz xload(const uint64_t* p) {
+  const g1j j = jload(p);
+  const fe zz = fq::q_mul(j.z, j.z);
+  return g1xyzz{j.x, j.y, zz, fq::q_mul(zz, j.z)};
+}
+static void xstore(uint64_t* o, uint32_t* raw, const g1xyzz& a) {
+  jstore(o, g1_xyzz_to_jacobian(a));
+  if (raw) {
+    memcpy(raw, a.x.l, 36);
+    memcpy(raw + 9, a.y.l, 36);
+    memcpy(raw + 18, a.zz.l, 36);
+    memcpy(raw + 27, a.zzz.l, 36);
+    raw[36] = g1_xyzz_is_identity(a) ? 1u : 0u;
+  }
+}
+
 extern "C" {
+// A chain of XYZZ operations on an accumulator (from the identity) and a
+// saved register, with points[k] (Jacobian words) the operand of step k:
+// op 0: acc = acc + p, 1: acc = p + acc, 2: acc = 2 acc (g1_xyzz_double),
+// 3: acc = acc + acc, 4: saved = acc, 5: acc = acc + saved, 6: acc = saved + acc.
+// out: Jacobian words of acc at the end; trace: after each step the raw
+// limbs of acc (X, Y, ZZ, ZZZ) and its identity flag (37 u32).
+void shim_g1_xyzz_chain(const uint64_t* points, const int* ops, int n, uint64_t* out, uint32_t* trace) {
+  g1xyzz acc = g1_xyzz_identity(), saved = g1_xyzz_identity();
+  for (int k = 0; k < n; ++k) {
+    const g1xyzz p = xload(points + 12 * k);
+    switch (ops[k]) {
+      case 0: acc = g1_xyzz_add(acc, p); break;
+      case 1: acc = g1_xyzz_add(p, acc); break;
+      case 2: g1_xyzz_double(acc); break;
+      case 3: acc = g1_xyzz_add(acc, acc); break;
+      case 4: saved = acc; break;
+      case 5: acc = g1_xyzz_add(acc, saved); break;
+      default: acc = g1_xyzz_add(saved, acc); break;
+    }
+    uint64_t scratch[12];
+    xstore(scratch, trace ? trace + 37 * k : nullptr, acc);
+  }
+  xstore(out, nullptr, acc);
+}
 void shim_fq_mul(const uint64_t* a, const uint64_t* b, uint64_t* o) { to64(o, fq::q_mul(from64(a), from64(b))); }
 void shim_fq_add(const uint64_t* a, const uint64_t* b, uint64_t* o) { to64(o, fq::q_add(from64(a), from64(b))); }
 void shim_fq_sub(const uint64_t* a, const uint64_t* b, uint64_t* o) { to64(o, fq::q_sub(from64(a), from64(b))); }

@@ -178,3 +178,69 @@ def test_bucket_sum(shim):
             assert _limb_value(row[18:27]) < 1.07 * Q and _limb_value(row[27:36]) < 1.07 * Q
             assert pow(ZZ, 3, Q) == pow(ZZZ, 2, Q)
             assert (X * pow(ZZ, -1, Q) % Q, Y * pow(ZZZ, -1, Q) % Q) == part
+
+
+def test_xyzz_chain(shim):
+    """The MSM's full XYZZ addition and doubling (g1_xyzz_add,
+    g1_xyzz_double: the merge, running-sum and window-sum formulas) over
+    random chains of operations, against affine Python arithmetic: additions
+    of fresh points with random Z in either operand order, of the
+    accumulator to itself (the doubling branch) and to an earlier value of
+    itself (running sums T += S), doublings, and the branches P + (-P) and
+    the identity on either side. After every step the accumulator holds the
+    invariant the kernels rely on: limbs 0..7 below 2^29, X and Y below 3q,
+    ZZ and ZZZ below 1.1q, ZZ^3 = ZZZ^2, and (X / ZZ, Y / ZZZ) the affine
+    point (ZZ = 0 exactly for the identity)."""
+    rnd = random.Random(11)
+    R = O.R_MOD
+    pool = [aff_mul(rnd.randrange(1, R), G) for _ in range(16)]
+    rinv = pow(1 << 261, -1, Q)
+    chains = []
+    for length in (5, 40, 200):
+        ops, pts = [], []
+        for _ in range(length):
+            op = rnd.choice([0, 0, 1, 2, 3, 4, 5, 6])
+            ops.append(op)
+            pts.append(rnd.choice(pool + [None]))
+        chains.append((ops, pts))
+    p = pool[0]
+    chains.append(([0, 0], [p, aff_neg(p)]))          # P + (-P): identity
+    chains.append(([0, 1], [p, p]))                   # P + P through addition
+    chains.append(([0, 4, 0, 6, 0], [p, None, p, None, aff_neg(p)]))
+    chains.append(([4, 0, 2, 3, 5, 0], [None, None, p, None, None, None]))
+    chains.append(([0, 4, 0, 0, 5], [p, None, aff_neg(p), p, None]))  # saved + acc with acc == saved
+    chains.append(([2, 3, 0], [None, None, None]))   # operations on the identity
+    for ops, pts in chains:
+        n = len(ops)
+        words = []
+        for q in pts:
+            words += list(jac(q, rnd.randrange(1, Q)))
+        out = U64x12()
+        trace = (ctypes.c_uint32 * (37 * n))()
+        shim.shim_g1_xyzz_chain((ctypes.c_uint64 * (12 * n))(*words), (ctypes.c_int * n)(*ops), n, out, trace)
+        rows = np.frombuffer(trace, dtype=np.uint32).reshape(n, 37)
+        acc = saved = None
+        for op, q, row in zip(ops, pts, rows):
+            if op == 0 or op == 1:
+                acc = aff_add(acc, q)
+            elif op == 2 or op == 3:
+                acc = aff_add(acc, acc)
+            elif op == 4:
+                saved = acc
+            else:
+                acc = aff_add(acc, saved)
+            assert bool(row[36]) == (acc is None), (ops, op)
+            vals = []
+            for c in range(4):
+                l = row[9 * c: 9 * c + 9]
+                assert all(int(v) < (1 << 29) for v in l[:8])
+                assert _limb_value(l) < 3 * Q
+                vals.append(_limb_value(l) * rinv % Q)
+            X, Y, ZZ, ZZZ = vals
+            assert _limb_value(row[18:27]) < 1.1 * Q and _limb_value(row[27:36]) < 1.1 * Q
+            assert pow(ZZ, 3, Q) == pow(ZZZ, 2, Q)
+            if acc is None:
+                assert ZZ == 0
+            else:
+                assert (X * pow(ZZ, -1, Q) % Q, Y * pow(ZZZ, -1, Q) % Q) == acc
+        assert aff(out) == acc

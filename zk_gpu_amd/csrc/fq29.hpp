@@ -388,31 +388,45 @@ ZK_HD void g1_bucket_add(g1xyzz& acc, bool& inf, const g1a& q, bool neg) {
 
 ZK_HD g1xyzz g1_xyzz_identity() { return g1xyzz{fq::q_zero(), fq::q_zero(), fq::q_zero(), fq::q_zero()}; }
 // the identity has ZZ = 0 (and ZZZ = 0); every other point has ZZ != 0 mod q
-ZK_HD bool g1_xyzz_is_identity(const g1xyzz& a) { return fq::q_is_zero(a.zz); }
+// (ZZ is always a normalized product or constant below 3q)
+ZK_HD bool g1_xyzz_is_identity(const g1xyzz& a) { return fq::q_is_zero3(a.zz); }
 
-// P1 + P2 in XYZZ coordinates, add-2008-s: 12M + 2S. Inputs and outputs:
-// X, Y normalized below 3q (q_sub's reduced results, or products);
-// ZZ, ZZZ normalized products below 1.1q. P == 0 means x1 == x2: then the
-// sum is 2 P1 (R == 0) or the identity.
+// P1 + P2 in XYZZ coordinates, add-2008-s: 12M + 2S, with lazy field steps.
+// Inputs and outputs: X, Y normalized below 3q; ZZ, ZZZ normalized products
+// below 1.1q. Bounds in units of q, as for g1_bucket_add:
+//   U1 = X1 ZZ2, U2 = X2 ZZ1, S1 = Y1 ZZZ2, S2 = Y2 ZZZ1 < 1.02
+//   P = U2 + 3q - U1 (normalized) < 4.02;  R = S2 + 3q - S1 (normalized) < 4.02
+//   PP < 1.1, PPP = P PP < 1.03, Q = U1 PP < 1.01, R^2 < 1.1, ZZ1 ZZ2 < 1.01
+//   X3 = R^2 + 3q - PPP + 3q - Q + 3q - Q (normalized) < 10.1
+//   Q + 12q - X3 < 13.01, R (Q - X3) < 1.31, S1 PPP < 1.01, Y3 < 4.31
+//   ZZ3 = ZZ1 ZZ2 PP, ZZZ3 = ZZZ1 ZZZ2 PPP < 1.01
+// X3 and Y3 are value-reduced below 3q. ZZ1 ZZ2 != 0, so ZZ3 == 0 exactly
+// when P == 0, i.e. x1 == x2: then the sum is 2 P1 (R == 0) or the identity.
 ZK_HD g1xyzz g1_xyzz_add(const g1xyzz& p, const g1xyzz& q) {
   using namespace fq;
   if (g1_xyzz_is_identity(p)) return q;
   if (g1_xyzz_is_identity(q)) return p;
   const fe U1 = q_mul(p.x, q.zz), U2 = q_mul(q.x, p.zz);
   const fe S1 = q_mul(p.y, q.zzz), S2 = q_mul(q.y, p.zzz);
-  const fe P = q_sub(U2, U1), R = q_sub(S2, S1);
-  if (q_is_zero(P)) {
+  const fe P = q_norm(q_sub_lazy<3>(U2, U1)), R = q_norm(q_sub_lazy<3>(S2, S1));
+  const fe PP = q_mul(P, P);
+  const fe ZZ3 = q_mul(q_mul(p.zz, q.zz), PP);
+  if (q_is_zero3(ZZ3)) {
     if (!q_is_zero(R)) return g1_xyzz_identity();
     g1xyzz d = p;
     g1_xyzz_double(d);
     return d;
   }
-  const fe PP = q_sqr(P), PPP = q_mul(P, PP), Q = q_mul(U1, PP);
+  const fe PPP = q_mul(P, PP), Q = q_mul(U1, PP);
+  fe X3 = q_norm(q_sub_lazy<3>(q_sub_lazy<3>(q_sub_lazy<3>(q_mul(R, R), PPP), Q), Q));
+  fe Y3 = q_norm(q_sub_lazy<3>(q_mul(q_sub_lazy<12>(Q, X3), R), q_mul(S1, PPP)));
   g1xyzz o;
-  o.x = q_sub(q_sub(q_sqr(R), PPP), q_dbl(Q));
-  o.y = q_sub(q_mul(R, q_sub(Q, o.x)), q_mul(S1, PPP));
-  o.zz = q_mul(q_mul(p.zz, q.zz), PP);
+  o.zz = ZZ3;
   o.zzz = q_mul(q_mul(p.zzz, q.zzz), PPP);
+  q_reduce(X3);
+  q_reduce(Y3);
+  o.x = X3;
+  o.y = Y3;
   return o;
 }
 
