@@ -30,6 +30,11 @@ namespace fq {
 constexpr uint32_t P29[NLIMBS] = {0x187cfd47u, 0x010460b6u, 0x1c72a34fu, 0x02d522d0u, 0x1585d978u,
                                   0x02db40c0u, 0x00a6e141u, 0x0e5c2634u, 0x0030644eu};
 constexpr uint32_t NINV29 = 0x04866389u;  // -q^-1 mod 2^29
+// q for the column-asm multiplier (mont_cols.hpp, device code)
+struct FqMod {
+  static constexpr uint32_t ninv = NINV29;
+  static constexpr uint32_t p(int i) { return P29[i]; }
+};
 // 32-bit words of q
 constexpr uint32_t P32[8] = {0xd87cfd47u, 0x3c208c16u, 0x6871ca8du, 0x97816a91u,
                              0x8181585du, 0xb85045b6u, 0xe131a029u, 0x30644e72u};
@@ -52,8 +57,13 @@ ZK_HD uint32_t p32(int i) {
        : i == 5 ? P32[5] : i == 6 ? P32[6] : P32[7];
 }
 
-// a * b * 2^-261 mod q (not reduced); contracts as zkgpu::mont_mul.
+// a * b * 2^-261 mod q (not reduced); contracts as zkgpu::mont_mul. On the
+// device each column is one chain of multiply-adds from the carry
+// (mont_mul_cols); the host computes the same product in plain C.
 ZK_HD fe q_mul(const fe& a, const fe& b) {
+#ifdef __HIP_DEVICE_COMPILE__
+  return mont_mul_cols<FqMod>(a, b);
+#endif
   uint32_t m[NLIMBS];
   fe o;
   uint64_t acc = 0;
