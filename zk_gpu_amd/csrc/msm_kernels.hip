@@ -566,7 +566,8 @@ __global__ void __launch_bounds__(256) msm_seg_order_kernel(const uint32_t* __re
 
 // One thread per bucket segment (a run of one bucket's entries), in the
 // order of msm_seg_order_kernel: the sum of its points, accumulated in XYZZ
-// coordinates, stored in Jacobian ones at the segment's index.
+// coordinates and stored so (in the bucket array for a bucket of one
+// segment, else at the segment's index).
 __global__ void __launch_bounds__(64) msm_buckets_kernel(const BaseLimbs* __restrict__ blimbs, const uint32_t* __restrict__ offsets,
                                    const uint32_t* __restrict__ counts, const uint32_t* __restrict__ list,
                                    const uint32_t* __restrict__ part_seg_off, const uint32_t* __restrict__ seg_local,
