@@ -16,8 +16,11 @@
 // has Z = 0) follow the same textbook formulas the reference uses
 // (cuda/curve.cuh:106-297): doubling dbl-2009-l, mixed addition
 // madd-2007-bl, general addition add-2007-bl, with the equal-x cases routed
-// to doubling or the identity. The MSM's bucket sums accumulate in XYZZ
-// coordinates instead (madd-2008-s, dbl-2008-s-1; g1_bucket_add).
+// to doubling or the identity. The MSM works in XYZZ coordinates instead
+// (x = X / ZZ, y = Y / ZZZ): mixed additions into the bucket sums
+// (madd-2008-s, g1_bucket_add), full additions and doublings of bucket,
+// segment, run and window sums (add-2008-s, dbl-2008-s-1); the Jacobian
+// formulas serve the synthetic bases and the host-side tests.
 #pragma once
 #include <stdint.h>
 
