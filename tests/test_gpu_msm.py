@@ -83,6 +83,26 @@ def test_vs_oracle_with_edge_bases(n):
     assert _msm_host(scalars, bases) == O.g1_msm(scalars, bases, threads=16)
 
 
+@pytest.mark.parametrize("n,width", [(100, 5), (1500, 9), (3000, 10), (10000, 11), (20000, 15)])
+def test_closed_form_every_window_width(n, width):
+    """random scalars at a size for each window width of the plan table that
+    the other tests leave out (widths 4, 8, 10, 15 and 16 come up there too),
+    against the closed form and, up to 3000 points, the oracle; widths whose
+    top window holds only a few of the scalars' bits, and the switch from 11
+    to 15 bits between 2^13 and 2^14 points"""
+    import ctypes
+    cw = ctypes.c_uint32(0)
+    Z.lib().zkgpu_msm_workspace_bytes(n, ctypes.byref(cw))
+    assert cw.value == width
+    a, d = 0x51 + width, 0x1F2E3D4C5B6A7988
+    bases = O.g1_arith_bases(n, a, d)
+    scalars = O.gen(4700 + width, n)
+    got = _msm_host(scalars, bases)
+    assert got == O.g1_mul_gen(_weighted_sum(scalars, a, d))
+    if n <= 3000:
+        assert got == O.g1_msm(scalars, bases)
+
+
 @pytest.mark.parametrize("n", [1 << 12, 1 << 16, 1 << 20, 700001, (1 << 21) + 4097])
 def test_closed_form_large(n):
     """random scalars; the odd sizes leave partial sort chunks, slices and
