@@ -46,8 +46,7 @@ hipError_t launch_msm_bucket_merge(void* seg_sums, const uint32_t* seg_bucket, c
                                    uint64_t max_segs, void* buckets, hipStream_t s);
 hipError_t launch_msm_chunks(const void* buckets, uint32_t half, uint32_t run, uint32_t runs_per_window,
                              uint32_t windows, void* parts, hipStream_t s);
-hipError_t launch_msm_windows(const void* parts, uint32_t runs_per_window, uint32_t windows, void* out,
-                              hipStream_t s);
+hipError_t launch_msm_windows(void* parts, uint32_t runs_per_window, uint32_t windows, void* out, hipStream_t s);
 size_t msm_base_limb_bytes();
 size_t msm_point_limb_bytes();
 hipError_t launch_fill_random_points(void* out, uint64_t count, uint64_t seed, uint64_t start, hipStream_t s);

@@ -690,13 +690,13 @@ __global__ void __launch_bounds__(64) msm_chunks_kernel(const PointLimbs* __rest
 // run); the window's sum goes out as canonical halo2 words X, Y, Z (96 bytes).
 constexpr uint32_t WINDOW_THREADS = 512;
 __global__ void __launch_bounds__(WINDOW_THREADS) msm_windows_kernel(const PointLimbs* __restrict__ parts,
-                                                                     uint32_t parts_per_window,
-                                                                     uint4* __restrict__ out) {
+                                                                     uint32_t parts_per_window, uint32_t count,
+                                                                     uint32_t stride, uint4* __restrict__ out) {
   __shared__ PointLimbs tree[WINDOW_THREADS];
   const uint32_t w = blockIdx.x;
   g1xyzz acc = g1_xyzz_identity();
-  for (uint32_t k = threadIdx.x; k < parts_per_window; k += WINDOW_THREADS)
-    acc = g1_xyzz_add(acc, load_point(parts, w * parts_per_window + k));
+  for (uint32_t k = threadIdx.x; k < count; k += WINDOW_THREADS)
+    acc = g1_xyzz_add(acc, load_point(parts, w * parts_per_window + k * stride));
   store_point(tree, threadIdx.x, acc);
   __syncthreads();
   for (uint32_t off = WINDOW_THREADS / 2; off > 0; off >>= 1) {
@@ -721,6 +721,35 @@ __global__ void __launch_bounds__(WINDOW_THREADS) msm_windows_kernel(const Point
     o[4] = make_uint4(z[0], z[1], z[2], z[3]);
     o[5] = make_uint4(z[4], z[5], z[6], z[7]);
   }
+}
+
+// Windows of many parts (WINDOW_SLICE_MIN and up): each window's parts are
+// first summed in slices by one-wave workgroups, spread over many CUs
+// (the sum is a chain of dependent additions, so its time is its depth).
+// Workgroup (w, g) sums slice g of window w (parts_per_window / slices
+// entries from g times that) through its 64 lanes and a tree in LDS and
+// leaves the sum in the slice's first entry (only this workgroup reads the
+// slice, and every read is done before that store).
+constexpr uint32_t WINDOW_SLICE_MIN = 4096;
+constexpr uint32_t WINDOW_SLICES = 32;
+__global__ void __launch_bounds__(64) msm_window_slices_kernel(PointLimbs* __restrict__ parts,
+                                                                uint32_t parts_per_window) {
+  __shared__ PointLimbs tree[64];
+  const uint32_t w = blockIdx.x / WINDOW_SLICES, g = blockIdx.x % WINDOW_SLICES;
+  const uint32_t len = parts_per_window / WINDOW_SLICES;
+  PointLimbs* slice = parts + (size_t)w * parts_per_window + (size_t)g * len;
+  g1xyzz acc = g1_xyzz_identity();
+  for (uint32_t k = threadIdx.x; k < len; k += 64) acc = g1_xyzz_add(acc, load_point(slice, k));
+  store_point(tree, threadIdx.x, acc);
+  __syncthreads();
+  for (uint32_t off = 32; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      acc = g1_xyzz_add(acc, load_point(tree, threadIdx.x + off));
+      store_point(tree, threadIdx.x, acc);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) store_point(slice, 0, acc);
 }
 
 // Synthetic bases: point i = k_i G for the generator G = (1, 2), k_i a
@@ -873,10 +902,18 @@ hipError_t launch_msm_chunks(const void* buckets, uint32_t half, uint32_t run, u
                                                           total, (PointLimbs*)parts);
   return hipGetLastError();
 }
-hipError_t launch_msm_windows(const void* parts, uint32_t runs_per_window, uint32_t windows, void* out,
-                              hipStream_t s) {
-  msm_windows_kernel<<<windows, WINDOW_THREADS, 0, s>>>((const PointLimbs*)parts, 2 * runs_per_window,
-                                                        (uint4*)out);
+// parts: overwritten where the windows are sliced (a slice's first entry
+// takes the slice's sum)
+hipError_t launch_msm_windows(void* parts, uint32_t runs_per_window, uint32_t windows, void* out, hipStream_t s) {
+  const uint32_t per_window = 2 * runs_per_window;  // a power of two
+  if (per_window >= WINDOW_SLICE_MIN) {
+    msm_window_slices_kernel<<<windows * WINDOW_SLICES, 64, 0, s>>>((PointLimbs*)parts, per_window);
+    msm_windows_kernel<<<windows, WINDOW_THREADS, 0, s>>>((const PointLimbs*)parts, per_window, WINDOW_SLICES,
+                                                          per_window / WINDOW_SLICES, (uint4*)out);
+  } else {
+    msm_windows_kernel<<<windows, WINDOW_THREADS, 0, s>>>((const PointLimbs*)parts, per_window, per_window, 1,
+                                                          (uint4*)out);
+  }
   return hipGetLastError();
 }
 size_t msm_base_limb_bytes() { return sizeof(BaseLimbs); }
