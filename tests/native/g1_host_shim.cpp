@@ -69,6 +69,11 @@ void shim_g1_xyzz_chain(const uint64_t* points, const int* ops, int n, uint64_t*
   }
   xstore(out, nullptr, acc);
 }
+// halo2 words (any value below 2^256) -> canonical words of 32 w mod q by
+// the shift form (shift = 1) or the product form (shift = 0)
+void shim_fq_from_words(const uint32_t* w, int shift, uint32_t* o) {
+  fq::q_canonical_words(o, shift ? fq::q_from_words_shift(w) : fq::q_from_words(w));
+}
 void shim_fq_mul(const uint64_t* a, const uint64_t* b, uint64_t* o) { to64(o, fq::q_mul(from64(a), from64(b))); }
 void shim_fq_add(const uint64_t* a, const uint64_t* b, uint64_t* o) { to64(o, fq::q_add(from64(a), from64(b))); }
 void shim_fq_sub(const uint64_t* a, const uint64_t* b, uint64_t* o) { to64(o, fq::q_sub(from64(a), from64(b))); }

@@ -244,3 +244,22 @@ def test_xyzz_chain(shim):
             else:
                 assert (X * pow(ZZ, -1, Q) % Q, Y * pow(ZZZ, -1, Q) % Q) == acc
         assert aff(out) == acc
+
+
+def test_base_conversion_by_shift(shim):
+    """The MSM's base records: halo2's x * 2^256 mod q words to the kernels'
+    x * 2^261 mod q as canonical words, by a 5-bit shift and a value
+    reduction (q_from_words_shift) and by a Montgomery product with 2^266
+    mod q (q_from_words): both equal 32 w mod q, for canonical words, words
+    at or above q (up to 2^256 - 1) and the edge values."""
+    rnd = random.Random(17)
+    vals = [0, 1, Q - 1, Q, Q + 1, 2 * Q - 1, (1 << 256) - 1, (1 << 255), (1 << 254) + 12345]
+    vals += [rnd.randrange(Q) for _ in range(300)] + [rnd.randrange(1 << 256) for _ in range(300)]
+    U32x8 = ctypes.c_uint32 * 8
+    for v in vals:
+        w = U32x8(*[(v >> (32 * i)) & 0xFFFFFFFF for i in range(8)])
+        want = 32 * v % Q
+        for shift in (1, 0):
+            o = U32x8()
+            shim.shim_fq_from_words(w, shift, o)
+            assert sum(int(o[i]) << (32 * i) for i in range(8)) == want, (hex(v), shift)

@@ -189,6 +189,18 @@ ZK_HD fe q_zero() {
 }
 // halo2 words (x * 2^256 mod q, < 2^256) -> small x * 2^261
 ZK_HD fe q_from_words(const uint32_t w[8]) { return q_mul(unpack_words(w), q_from_const(TO261)); }
+// The same residue, x * 2^261 = 32 (x * 2^256) mod q, by a 5-bit shift of
+// the limbs (value below 2^261) and one value reduction instead of a
+// product: normalized out, value below 3q.
+ZK_HD fe q_from_words_shift(const uint32_t w[8]) {
+  fe a = unpack_words(w);  // limbs 0..7 < 2^29, top limb < 2^24
+  a.l[NLIMBS - 1] = (a.l[NLIMBS - 1] << 5) | (a.l[NLIMBS - 2] >> (LIMB_BITS - 5));
+#pragma unroll
+  for (int i = NLIMBS - 2; i >= 1; --i) a.l[i] = ((a.l[i] << 5) | (a.l[i - 1] >> (LIMB_BITS - 5))) & LIMB_MASK;
+  a.l[0] = (a.l[0] << 5) & LIMB_MASK;
+  q_reduce(a);
+  return a;
+}
 // small x * 2^261 -> canonical halo2 words x * 2^256 mod q
 ZK_HD void q_to_words(uint32_t w[8], const fe& a) { q_canonical_words(w, q_mul(a, q_from_const(R256))); }
 

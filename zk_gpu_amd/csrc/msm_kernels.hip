@@ -194,8 +194,8 @@ __global__ void __launch_bounds__(256) msm_digits_kernel(const uint4* __restrict
     for (int k = 0; k < 8; ++k) any |= xw[k] | yw[k];
     const bool inf = any == 0;  // halo2's affine identity: x = y = 0
     uint32_t xc[8], yc[8];
-    fq::q_canonical_words(xc, fq::q_from_words(xw));
-    fq::q_canonical_words(yc, fq::q_from_words(yw));
+    fq::q_canonical_words(xc, fq::q_from_words_shift(xw));
+    fq::q_canonical_words(yc, fq::q_from_words_shift(yw));
     BaseLimbs e;
     e.x0 = make_uint4(xc[0], xc[1], xc[2], xc[3]);
     e.x1 = make_uint4(xc[4], xc[5], xc[6], xc[7]);
